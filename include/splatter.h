@@ -1,0 +1,179 @@
+/*
+ * splatter.h -- C ABI of the MI355X-native splat pipeline.
+ *
+ * Drop-in for the per-frame path of Lichtso/splatter: every entry point below
+ * names the Rust item it replaces.  A Rust crate keeps the reference's public
+ * API (`splatter::renderer::{Renderer, Configuration, DepthSorting}`,
+ * `splatter::scene::Scene`) and forwards to these functions; the binding is
+ * spelled out in INTEGRATION.md.
+ *
+ * Conventions
+ *  - Opaque handles; every call returns an int status (SPL_OK == 0, negative
+ *    on error).  spl_last_error() returns the calling thread's last message.
+ *    No C++ exception crosses the ABI.  (The reference panics instead:
+ *    scene.rs:202-224 unwraps, renderer.rs:90 asserts.)
+ *  - Matrices are 16 floats, column-major, column c = (x, y, z, w) at
+ *    [4c, 4c+4): the layout of `[ppga3d::Point; 4]` produced by
+ *    utils.rs:31-42 motor3d_to_mat4.
+ *  - Device work is asynchronous on the device's stream, like wgpu's
+ *    queue.submit (renderer.rs:465); spl_synchronize() waits for it.
+ *  - One renderer renders one frame at a time (the reference shares its
+ *    sorting/entry buffers across scenes the same way, renderer.rs:79-83).
+ *    Scenes are read-only during rendering and may be shared by renderers
+ *    of the same device.
+ */
+#ifndef SPLATTER_H
+#define SPLATTER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spl_device spl_device;
+typedef struct spl_renderer spl_renderer;
+typedef struct spl_scene spl_scene;
+typedef struct spl_frame spl_frame;
+
+enum {
+    SPL_OK = 0,
+    SPL_ERROR_INVALID_ARGUMENT = -1,
+    SPL_ERROR_OUT_OF_MEMORY = -2,
+    SPL_ERROR_DEVICE = -3,
+    SPL_ERROR_UNSUPPORTED = -4,
+    SPL_ERROR_IO = -5,
+    SPL_ERROR_CAPACITY = -6
+};
+
+/* renderer.rs:14-23 DepthSorting.  NONE draws in splat order and culls with
+ * the vertex-stage predicate (shaders.wgsl:521-524).  CPU, GPU and
+ * GPU_INDIRECT_DRAW all produce the CPU fallback's visible set and stable
+ * order (renderer.rs:372-396), computed on the GPU. */
+typedef enum {
+    SPL_DEPTH_SORTING_NONE = 0,
+    SPL_DEPTH_SORTING_CPU = 1,
+    SPL_DEPTH_SORTING_GPU = 2,
+    SPL_DEPTH_SORTING_GPU_INDIRECT_DRAW = 3
+} spl_depth_sorting;
+
+/* Replaces surface_configuration, of which only .format is read
+ * (renderer.rs:259); the showcase uses Bgra8Unorm (application_framework.rs:184). */
+typedef enum {
+    SPL_FRAME_FORMAT_BGRA8_UNORM = 0,
+    SPL_FRAME_FORMAT_RGBA8_UNORM = 1
+} spl_frame_format;
+
+/* UNORM8 reproduces an 8-bit unorm render target: the source is clamped to
+ * [0,1] and the destination re-quantised after every blend.  F32 keeps the
+ * running colour and transmittance in f32 and quantises once at the end. */
+typedef enum {
+    SPL_BLEND_UNORM8 = 0,
+    SPL_BLEND_F32 = 1
+} spl_blend_mode;
+
+/* renderer.rs:26-47 Configuration, field by field, plus extensions. */
+typedef struct spl_configuration {
+    spl_frame_format frame_format;        /* surface_configuration.format */
+    spl_depth_sorting depth_sorting;      /* depth_sorting */
+    int32_t use_covariance_for_scale;     /* use_covariance_for_scale */
+    int32_t use_unaligned_rectangles;     /* use_unaligned_rectangles */
+    uint32_t spherical_harmonics_order;   /* spherical_harmonics_order, 0..=3 */
+    uint64_t max_splat_count;             /* max_splat_count */
+    uint32_t radix_bits_per_digit;        /* radix_bits_per_digit: 8 (default) or 10 */
+    float frustum_culling_tolerance;      /* frustum_culling_tolerance */
+    float ellipse_margin;                 /* ellipse_margin */
+    float splat_scale;                    /* splat_scale */
+    /* extensions */
+    float field_of_view_y;                /* radians; the reference hard-codes PI/2 (renderer.rs:366) */
+    spl_blend_mode blend_mode;
+    uint32_t tile_pairs_per_splat;        /* capacity of (splat, screen tile) pairs, per max_splat_count */
+} spl_configuration;
+
+const char* spl_last_error(void);
+
+/* The showcase's Configuration (showcase.rs:39-50) with field_of_view_y = PI/2. */
+void spl_configuration_default(spl_configuration* config);
+
+/* wgpu::Device + wgpu::Queue (application_framework.rs:167-177): one HIP
+ * device and one stream. */
+int spl_device_open(int ordinal, spl_device** out);
+int spl_device_close(spl_device* device);
+int spl_synchronize(spl_device* device);
+
+/* renderer.rs:88 Renderer::new: allocates the sorting, entry and record
+ * workspace for config->max_splat_count splats. */
+int spl_renderer_create(spl_device* device, const spl_configuration* config, spl_renderer** out);
+int spl_renderer_destroy(spl_renderer* renderer);
+int spl_renderer_get_config(const spl_renderer* renderer, spl_configuration* out);
+
+/* scene.rs:45 Scene::new: splat_count is clamped to max_splat_count (scene.rs:46). */
+int spl_scene_create(spl_renderer* renderer, uint64_t splat_count, spl_scene** out);
+/* scene.rs:158 Scene::new_dev_test */
+int spl_scene_new_dev_test(spl_renderer* renderer, spl_scene** out);
+/* scene.rs:40 pub splat_count */
+int spl_scene_splat_count(const spl_scene* scene, uint64_t* out);
+int spl_scene_destroy(spl_scene* scene);
+
+/* scene.rs:197 Scene::parse_file_header: header length in bytes and the
+ * "element vertex" count of a 3DGS PLY file. */
+int spl_parse_file_header(const char* path, uint16_t* header_size, uint64_t* splat_count);
+/* scene.rs:216 Scene::load_chunk: decodes splats [begin, end) of the file
+ * (end clamped to splat_count, scene.rs:217) into the scene. */
+int spl_scene_load_chunk(spl_scene* scene, const char* path, uint16_t header_size,
+                         uint64_t begin, uint64_t end);
+/* The decode of scene.rs:225-255 applied to `count` in-memory 248-byte
+ * SerializedSplat records (scene.rs:17-24) written at splat index `first`. */
+int spl_scene_load_records(spl_scene* scene, uint64_t first, uint64_t count, const void* records);
+
+/* The render target: replaces the wgpu::TextureView + viewport_size pair
+ * (renderer.rs:359-360).  4 bytes per pixel in the renderer's frame_format. */
+int spl_frame_create(spl_device* device, uint32_t width, uint32_t height, spl_frame** out);
+int spl_frame_destroy(spl_frame* frame);
+/* Waits for the device and copies width*height*4 bytes to host memory. */
+int spl_frame_read(spl_frame* frame, void* host, uint64_t size);
+
+/* renderer.rs:355 Renderer::render_frame.  camera_matrix and view_matrix
+ * are motor3d_to_mat4(motor) and motor3d_to_mat4(motor.inverse())
+ * (renderer.rs:364-365); the viewport is the frame's size. */
+int spl_render_frame(spl_renderer* renderer, spl_scene* scene, spl_frame* frame,
+                     const float camera_matrix[16], const float view_matrix[16]);
+
+/* Batched form (no counterpart in the reference): renders `view_count`
+ * views of one scene, view i with matrices at camera_matrices + 16*i and
+ * view_matrices + 16*i into frames[i].  Same outputs as view_count calls of
+ * spl_render_frame. */
+int spl_render_views(spl_renderer* renderer, spl_scene* scene, uint32_t view_count,
+                     spl_frame* const* frames, const float* camera_matrices,
+                     const float* view_matrices);
+
+/* ---- parity taps (replace the commented readbacks, renderer.rs:456-464) - */
+/* Visible count of the last frame (the indirect draw's instance_count,
+ * shaders.wgsl:458-461). */
+int spl_debug_visible_count(spl_renderer* renderer, uint64_t* out);
+/* Sorted (key, splat index) entries of the last frame: entry buffer A of
+ * renderer.rs:395 in CPU mode.  Writes min(capacity, visible) entries. */
+int spl_debug_read_entries(spl_renderer* renderer, uint32_t* keys, uint32_t* indices,
+                           uint64_t capacity, uint64_t* count);
+/* The last frame's per-splat vertex outputs in draw order, 10 floats each:
+ * rgb, alpha, affine col0.xy, col1.xy, translation.xy. */
+int spl_debug_read_records(spl_renderer* renderer, float* records, uint64_t capacity, uint64_t* count);
+/* When enabled, the next frames also store each visible splat's contour
+ * conic M (shaders.wgsl:165) in FP64: xx, yy, zz, xy, xz, yz, in draw order. */
+int spl_debug_enable_conics(spl_renderer* renderer, int enable);
+int spl_debug_read_conics(spl_renderer* renderer, double* conics, uint64_t capacity, uint64_t* count);
+/* Number of (splat, screen tile) pairs of the last frame. */
+int spl_debug_pair_count(spl_renderer* renderer, uint64_t* out);
+
+/* ---- timing taps for the harness ---------------------------------------- */
+/* Per-stage device time of the last frame in milliseconds, measured with HIP
+ * events on the renderer's stream when enabled: cull, sort, project, bin,
+ * pair sort, ranges, raster (7 values). */
+int spl_debug_enable_timing(spl_renderer* renderer, int enable);
+int spl_debug_stage_times(spl_renderer* renderer, float* ms, uint32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

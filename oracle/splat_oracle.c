@@ -1,0 +1,473 @@
+/*
+ * splat_oracle.c -- CPU restatement of splatter's per-frame splat pipeline.
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product); see
+ * splat_oracle.h for the pins of each function.  Build: oracle/Makefile
+ * (gcc -O2 -ffp-contract=off).
+ */
+#include "splat_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* ---------------------------------------------------------------------- */
+/* scene.rs                                                                */
+/* ---------------------------------------------------------------------- */
+
+#define MAX_SIZE_VARIANCE 5.0f /* scene.rs:13 */
+
+/* scene.rs:225-249 (load_chunk decode loop).  Rotor::signum comes from
+ * geometric_algebra 0.3.0 (not vendored): restated as r / sqrt(r.r) with the
+ * squared magnitude summed left to right -- parity unpinned at that
+ * third-party boundary. */
+void orc_decode_serialized(const orc_serialized_splat* in, size_t count, orc_splat* out) {
+    for (size_t i = 0; i < count; ++i) {
+        const orc_serialized_splat* s = &in[i];
+        orc_splat* d = &out[i];
+        const float* r = s->rotation;
+        float sq = ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]) + r[3] * r[3];
+        float inv = 1.0f / sqrtf(sq);
+        for (int k = 0; k < 4; ++k) d->rotation[k] = r[k] * inv;
+        for (int k = 0; k < 3; ++k) d->center[k] = s->center[k];
+        d->padding = 0.0f;
+        float average = ((s->scale[0] + s->scale[1]) + s->scale[2]) / 3.0f;
+        for (int k = 0; k < 3; ++k)
+            d->scale[k] = expf(fminf(fmaxf(s->scale[k], average - MAX_SIZE_VARIANCE),
+                                     average + MAX_SIZE_VARIANCE));
+        d->alpha = 1.0f / (1.0f + expf(-s->alpha));
+        memcpy(d->sh, s->color, sizeof(d->sh));
+    }
+}
+
+/* scene.rs:157-192 Scene::new_dev_test. */
+void orc_dev_test_scene(orc_splat out[3]) {
+    memset(out, 0, 3 * sizeof(orc_splat));
+    for (int i = 0; i < 3; ++i) {
+        out[i].rotation[0] = 1.0f;
+        out[i].center[0] = i == 0 ? 1.0f : 0.0f;
+        out[i].center[1] = i == 1 ? 1.0f : 0.0f;
+        out[i].center[2] = i == 2 ? 1.0f : 0.0f;
+        out[i].scale[0] = 0.2f;
+        out[i].scale[1] = 0.1f;
+        out[i].scale[2] = 0.05f;
+        out[i].alpha = 1.0f;
+        out[i].sh[0] = i == 0 ? 1.0f : 0.0f;
+        out[i].sh[1] = i == 1 ? 1.0f : 0.0f;
+        out[i].sh[2] = i == 2 ? 1.0f : 0.0f;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* utils.rs / renderer.rs camera                                           */
+/* ---------------------------------------------------------------------- */
+
+/* utils.rs:45-55 */
+void orc_perspective_projection(float width, float height, float near_, float far_, float out[16]) {
+    float denominator = 1.0f / (near_ - far_);
+    memset(out, 0, 16 * sizeof(float));
+    out[0] = 1.0f / width;
+    out[5] = 1.0f / height;
+    out[10] = -far_ * denominator;
+    out[11] = 1.0f;
+    out[14] = (near_ * far_) * denominator;
+}
+
+/* utils.rs:58-65: column j = ((a0*b[j][0] + a1*b[j][1]) + a2*b[j][2]) + a3*b[j][3] */
+void orc_mat4_multiplication(const float a[16], const float b[16], float out[16]) {
+    float tmp[16];
+    for (int j = 0; j < 4; ++j)
+        for (int r = 0; r < 4; ++r)
+            tmp[4 * j + r] = ((a[r] * b[4 * j + 0] + a[4 + r] * b[4 * j + 1]) +
+                              a[8 + r] * b[4 * j + 2]) + a[12 + r] * b[4 * j + 3];
+    memcpy(out, tmp, sizeof(tmp));
+}
+
+/* utils.rs:68-70 */
+void orc_mat4_transform(const float a[16], const float b[4], float out[4]) {
+    for (int r = 0; r < 4; ++r)
+        out[r] = ((a[r] * b[0] + a[4 + r] * b[1]) + a[8 + r] * b[2]) + a[12 + r] * b[3];
+}
+
+/* renderer.rs:364-370,397-409.  field_of_view_y replaces the hard-coded
+ * PI*0.5 (renderer.rs:366) and view_width = aspect * view_height replaces
+ * aspect / view_height (renderer.rs:368); at the reference's 90 degrees
+ * tanf gives exactly 1.0f so both are bit-identical there. */
+void orc_camera_uniforms(const float camera_matrix[16], const float view_matrix[16],
+                         uint32_t width, uint32_t height, float field_of_view_y,
+                         float frustum_culling_tolerance, float ellipse_margin,
+                         float splat_scale, orc_uniforms* out) {
+    float view_height = tanf(field_of_view_y * 0.5f);
+    float view_width = ((float)width / (float)height) * view_height;
+    float projection[16];
+    orc_perspective_projection(view_width, view_height, 1.0f, 1000.0f, projection);
+    memcpy(out->camera_matrix, camera_matrix, 16 * sizeof(float));
+    memcpy(out->view_matrix, view_matrix, 16 * sizeof(float));
+    orc_mat4_multiplication(projection, view_matrix, out->view_projection_matrix);
+    out->view_size[0] = view_width;
+    out->view_size[1] = view_height;
+    out->image_size[0] = width;
+    out->image_size[1] = height;
+    out->frustum_culling_tolerance = frustum_culling_tolerance;
+    out->ellipse_size_bias = (0.2f * view_width) / (float)width;
+    out->ellipse_margin = ellipse_margin;
+    out->splat_scale = splat_scale;
+}
+
+/* ---------------------------------------------------------------------- */
+/* renderer.rs:372-396 CPU depth-sort fallback                             */
+/* ---------------------------------------------------------------------- */
+
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* Bottom-up stable merge sort on (key, value) by key. */
+void orc_stable_sort_pairs(uint32_t* keys, uint32_t* values, size_t n, uint32_t* scratch) {
+    uint32_t *sk = keys, *sv = values, *dk = scratch, *dv = scratch + n;
+    for (size_t width = 1; width < n; width *= 2) {
+        for (size_t lo = 0; lo < n; lo += 2 * width) {
+            size_t mid = lo + width < n ? lo + width : n;
+            size_t hi = lo + 2 * width < n ? lo + 2 * width : n;
+            size_t i = lo, j = mid, o = lo;
+            while (i < mid && j < hi) {
+                if (sk[j] < sk[i]) { dk[o] = sk[j]; dv[o++] = sv[j++]; }
+                else { dk[o] = sk[i]; dv[o++] = sv[i++]; }
+            }
+            while (i < mid) { dk[o] = sk[i]; dv[o++] = sv[i++]; }
+            while (j < hi) { dk[o] = sk[j]; dv[o++] = sv[j++]; }
+        }
+        uint32_t* t;
+        t = sk; sk = dk; dk = t;
+        t = sv; sv = dv; dv = t;
+    }
+    if (sk != keys) {
+        memcpy(keys, sk, n * sizeof(uint32_t));
+        memcpy(values, sv, n * sizeof(uint32_t));
+    }
+}
+
+size_t orc_cpu_depth_sort(const float* positions, size_t count, const float vp[16],
+                          float tol, uint32_t* keys, uint32_t* indices) {
+    size_t v = 0;
+    for (size_t i = 0; i < count; ++i) {
+        float p[4] = {positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 1.0f};
+        float h[4];
+        orc_mat4_transform(vp, p, h);
+        float r = 1.0f / h[3];
+        float c0 = h[0] * r, c1 = h[1] * r, c2 = h[2] * r;
+        if (fabsf(c0) < tol && fabsf(c1) < tol && fabsf(c2 - 0.5f) < 0.5f) {
+            keys[v] = f2u(c2);
+            indices[v] = (uint32_t)i;
+            ++v;
+        }
+    }
+    uint32_t* scratch = (uint32_t*)malloc(2 * (v ? v : 1) * sizeof(uint32_t));
+    orc_stable_sort_pairs(keys, indices, v, scratch);
+    free(scratch);
+    return v;
+}
+
+/* shaders.wgsl:54-61 worldToClipSpace + isInFrustum, used at vertex time
+ * (shaders.wgsl:521-524) when DepthSorting::None. */
+size_t orc_gpu_predicate_cull(const float* positions, size_t count, const float vp[16],
+                              float tol, uint32_t* indices) {
+    size_t v = 0;
+    for (size_t i = 0; i < count; ++i) {
+        float p[4] = {positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], 1.0f};
+        float h[4];
+        orc_mat4_transform(vp, p, h);
+        float w = h[3] + 0.0000001f;
+        float c0 = h[0] / w, c1 = h[1] / w, c2 = h[2] / w;
+        if (fabsf(c0) < tol && fabsf(c1) < tol && fabsf(c2 - 0.5f) < 0.5f) indices[v++] = (uint32_t)i;
+    }
+    return v;
+}
+
+/* ---------------------------------------------------------------------- */
+/* shaders.wgsl:248-294 spherical harmonics (f32)                          */
+/* ---------------------------------------------------------------------- */
+
+static const float kShc[16] = {
+    0.28209479177387814f, -0.4886025119029199f, 0.4886025119029199f, -0.4886025119029199f,
+    1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f, -1.0925484305920792f,
+    0.5462742152960396f, -0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+    0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f, -0.5900435899266435f,
+};
+
+float orc_sh_constant(int k) { return kShc[k]; }
+
+void orc_sh_color(const float d[3], const float sh[48], int order, float out[3]) {
+    float x = d[0], y = d[1], z = d[2];
+    float xx = x * x, yy = y * y, zz = z * z;
+    for (int c = 0; c < 3; ++c) {
+        float col = 0.5f;
+        col = col + kShc[0] * sh[c];
+        if (order > 0) {
+            col = col + (kShc[1] * sh[3 + c]) * y;
+            col = col + (kShc[2] * sh[6 + c]) * z;
+            col = col + (kShc[3] * sh[9 + c]) * x;
+        }
+        if (order > 1) {
+            col = col + ((kShc[4] * sh[12 + c]) * x) * y;
+            col = col + ((kShc[5] * sh[15 + c]) * y) * z;
+            col = col + ((kShc[6] * sh[18 + c]) * ((2.0f * zz - xx) - yy));
+            col = col + ((kShc[7] * sh[21 + c]) * x) * z;
+            col = col + (kShc[8] * sh[24 + c]) * (xx - yy);
+        }
+        if (order > 2) {
+            col = col + ((kShc[9] * sh[27 + c]) * y) * (3.0f * xx - yy);
+            col = col + (((kShc[10] * sh[30 + c]) * x) * y) * z;
+            col = col + ((kShc[11] * sh[33 + c]) * y) * ((4.0f * zz - xx) - yy);
+            col = col + ((kShc[12] * sh[36 + c]) * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+            col = col + ((kShc[13] * sh[39 + c]) * x) * ((4.0f * zz - xx) - yy);
+            col = col + ((kShc[14] * sh[42 + c]) * z) * (xx - yy);
+            col = col + ((kShc[15] * sh[45 + c]) * x) * (xx - 3.0f * yy);
+        }
+        out[c] = col;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* projection (shaders.wgsl:63-246, 530-547) in f64 and f32                */
+/* ---------------------------------------------------------------------- */
+
+#define REAL double
+#define SUFFIX _f64
+#define SQRT sqrt
+#define FABS fabs
+#include "project.inc"
+#undef REAL
+#undef SUFFIX
+#undef SQRT
+#undef FABS
+
+#define REAL float
+#define SUFFIX _f32
+#define SQRT sqrtf
+#define FABS fabsf
+#include "project.inc"
+#undef REAL
+#undef SUFFIX
+#undef SQRT
+#undef FABS
+
+void orc_project_f64(const orc_splat* splat, const orc_uniforms* u, int sh_order,
+                     int use_covariance_for_scale, orc_record* out, double* conic_out) {
+    project_f64(splat, u, sh_order, use_covariance_for_scale, out, conic_out);
+}
+
+void orc_project_f32(const orc_splat* splat, const orc_uniforms* u, int sh_order,
+                     int use_covariance_for_scale, orc_record* out, float* conic_out) {
+    project_f32(splat, u, sh_order, use_covariance_for_scale, out, conic_out);
+}
+
+void orc_contour_f64(const double scale[3], const double rotation[4], const double translation[3],
+                     const orc_uniforms* u, double M[9]) {
+    double m[3][3];
+    contour_f64(scale, rotation, translation, u, m);
+    memcpy(M, m, sizeof(m));
+}
+
+/* ---------------------------------------------------------------------- */
+/* fragment (shaders.wgsl:580-592) + fixed-function raster and blend       */
+/* ---------------------------------------------------------------------- */
+
+/* exp(x) by Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, and the
+ * degree-7 Taylor polynomial of e^r in Horner form, scaled by 2^k through
+ * the exponent field.  Only IEEE add/mul/floor, so it rounds identically on
+ * any IEEE-754 machine without contraction. */
+float orc_exp(float x) {
+    if (!(x > -87.0f)) return 0.0f;
+    if (x > 88.0f) return INFINITY;
+    float kf = floorf(x * 1.44269502f + 0.5f);
+    float r = (x - kf * 0.693145752f) - kf * 1.42860677e-06f;
+    float p = 0.000198412701f;
+    p = p * r + 0.00138888892f;
+    p = p * r + 0.00833333377f;
+    p = p * r + 0.0416666679f;
+    p = p * r + 0.166666672f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    int k = (int)kf;
+    return p * u2f((uint32_t)(k + 127) << 23);
+}
+
+static inline uint8_t q8(float x) {
+    x = fminf(fmaxf(x, 0.0f), 1.0f);
+    return (uint8_t)floorf(x * 255.0f + 0.5f);
+}
+
+typedef struct {
+    float i00, i01, i10, i11; /* inverse of the 2x2 affine part */
+    float tx, ty;             /* translation */
+    float half;               /* coverage half-extent: margin (unaligned) or radius*margin */
+    float r, g, b, a;
+    int valid;
+} orc_prepared;
+
+/* shaders.wgsl:554-571: the rectangle's inverse map.  The fragment's
+ * interpolated texture coordinate equals inverse(A) * (q - t) because the
+ * vertex positions are an affine image of the texture coordinates. */
+static void prepare(const orc_record* rec, float margin, int unaligned, orc_prepared* p) {
+    float A00 = rec->affine[0], A10 = rec->affine[1], A01 = rec->affine[2], A11 = rec->affine[3];
+    p->valid = 1;
+    for (int i = 0; i < 6; ++i)
+        if (!isfinite(rec->affine[i])) p->valid = 0;
+    float det = A00 * A11 - A10 * A01;
+    float rdet = 1.0f / det;
+    p->i00 = A11 * rdet;
+    p->i10 = -A10 * rdet;
+    p->i01 = -A01 * rdet;
+    p->i11 = A00 * rdet;
+    p->tx = rec->affine[4];
+    p->ty = rec->affine[5];
+    if (unaligned) {
+        p->half = margin;
+    } else {
+        float d0 = A00 * A00 + A10 * A10, d1 = A01 * A01 + A11 * A11;
+        p->half = sqrtf(fmaxf(d0, d1)) * margin;
+    }
+    p->r = rec->color[0]; p->g = rec->color[1]; p->b = rec->color[2]; p->a = rec->color[3];
+}
+
+/* Pixel centre -> view-plane coordinate (inverse of the viewport transform
+ * of clip positions (T*(tc,1))/view_size, shaders.wgsl:561). */
+static inline float pixel_qx(int px, float w, float vw) { return ((((float)px + 0.5f) / w) * 2.0f - 1.0f) * vw; }
+static inline float pixel_qy(int py, float h, float vh) { return (1.0f - (((float)py + 0.5f) / h) * 2.0f) * vh; }
+
+/* One fragment: coverage, alpha (shaders.wgsl:585-589) and blend
+ * (renderer.rs:260-271: colour = src*DstAlpha + dst, alpha = dst*(1-srcA)). */
+static inline void shade(const orc_prepared* p, int unaligned, float qx, float qy,
+                         int blend_mode, float* st) {
+    float dx = qx - p->tx, dy = qy - p->ty;
+    float tcx = p->i00 * dx + p->i01 * dy;
+    float tcy = p->i10 * dx + p->i11 * dy;
+    float h = p->half;
+    int inside = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
+                           : (dx >= -h && dx < h && dy >= -h && dy < h);
+    if (!inside) return;
+    float power = tcx * tcx + tcy * tcy;
+    float alpha = p->a * orc_exp(-0.5f * power);
+    if (!(alpha >= 1.0f / 255.0f)) return;
+    float sr = p->r * alpha, sg = p->g * alpha, sb = p->b * alpha;
+    if (blend_mode == ORC_BLEND_UNORM8) {
+        /* st holds the stored unorm8 values as floats 0..255 */
+        float da = st[3] / 255.0f;
+        float sa = fminf(fmaxf(alpha, 0.0f), 1.0f);
+        float s[3] = {fminf(fmaxf(sr, 0.0f), 1.0f), fminf(fmaxf(sg, 0.0f), 1.0f), fminf(fmaxf(sb, 0.0f), 1.0f)};
+        for (int c = 0; c < 3; ++c) st[c] = (float)q8(s[c] * da + st[c] / 255.0f);
+        st[3] = (float)q8(da * (1.0f - sa));
+    } else {
+        float da = st[3];
+        st[0] = sr * da + st[0];
+        st[1] = sg * da + st[1];
+        st[2] = sb * da + st[2];
+        st[3] = da * (1.0f - alpha);
+    }
+}
+
+void orc_raster(const orc_record* records, size_t count, const orc_uniforms* u,
+                int unaligned, int blend_mode, uint8_t* image, int brute_force) {
+    int W = (int)u->image_size[0], H = (int)u->image_size[1];
+    float vw = u->view_size[0], vh = u->view_size[1];
+    float margin = u->ellipse_margin;
+    float* state = (float*)malloc((size_t)W * H * 4 * sizeof(float));
+    for (size_t i = 0; i < (size_t)W * H; ++i) {
+        state[4 * i + 0] = 0.0f;
+        state[4 * i + 1] = 0.0f;
+        state[4 * i + 2] = 0.0f;
+        state[4 * i + 3] = blend_mode == ORC_BLEND_UNORM8 ? 255.0f : 1.0f;
+    }
+    for (size_t s = 0; s < count; ++s) {
+        orc_prepared p;
+        prepare(&records[s], margin, unaligned, &p);
+        if (!p.valid) continue;
+        int x0 = 0, x1 = W - 1, y0 = 0, y1 = H - 1;
+        if (!brute_force) {
+            /* conservative pixel box of the covered region, in double */
+            double ex, ey;
+            if (unaligned) {
+                ex = (fabs((double)records[s].affine[0]) + fabs((double)records[s].affine[2])) * margin;
+                ey = (fabs((double)records[s].affine[1]) + fabs((double)records[s].affine[3])) * margin;
+            } else {
+                ex = ey = p.half;
+            }
+            double cx = ((double)p.tx / vw + 1.0) * 0.5 * W, cy = (1.0 - (double)p.ty / vh) * 0.5 * H;
+            double rx = ex / vw * 0.5 * W + 2.0, ry = ey / vh * 0.5 * H + 2.0;
+            double fx0 = floor(cx - rx), fx1 = ceil(cx + rx), fy0 = floor(cy - ry), fy1 = ceil(cy + ry);
+            if (fx1 < 0 || fy1 < 0 || fx0 > W - 1 || fy0 > H - 1) continue;
+            x0 = fx0 < 0 ? 0 : (int)fx0;
+            y0 = fy0 < 0 ? 0 : (int)fy0;
+            x1 = fx1 > W - 1 ? W - 1 : (int)fx1;
+            y1 = fy1 > H - 1 ? H - 1 : (int)fy1;
+        }
+        for (int py = y0; py <= y1; ++py) {
+            float qy = pixel_qy(py, (float)H, vh);
+            for (int px = x0; px <= x1; ++px) {
+                float* st = &state[4 * ((size_t)py * W + px)];
+                /* transmittance 0: every later fragment adds src * 0 */
+                if (st[3] == 0.0f) continue;
+                shade(&p, unaligned, pixel_qx(px, (float)W, vw), qy, blend_mode, st);
+            }
+        }
+    }
+    for (size_t i = 0; i < (size_t)W * H; ++i)
+        for (int c = 0; c < 4; ++c)
+            image[4 * i + c] = blend_mode == ORC_BLEND_UNORM8 ? (uint8_t)state[4 * i + c] : q8(state[4 * i + c]);
+    free(state);
+}
+
+size_t orc_render_frame(const orc_splat* splats, size_t count, const orc_uniforms* u,
+                        int sh_order, int use_covariance_for_scale,
+                        int use_unaligned_rectangles, int blend_mode,
+                        uint32_t* keys, uint32_t* indices, orc_record* records,
+                        uint8_t* image) {
+    float* pos = (float*)malloc((count ? count : 1) * 3 * sizeof(float));
+    for (size_t i = 0; i < count; ++i)
+        for (int k = 0; k < 3; ++k) pos[3 * i + k] = splats[i].center[k];
+    size_t v = orc_cpu_depth_sort(pos, count, u->view_projection_matrix, u->frustum_culling_tolerance,
+                                  keys, indices);
+    free(pos);
+    for (size_t r = 0; r < v; ++r)
+        orc_project_f64(&splats[indices[r]], u, sh_order, use_covariance_for_scale, &records[r], NULL);
+    if (image) orc_raster(records, v, u, use_unaligned_rectangles, blend_mode, image, 0);
+    return v;
+}
+
+/* ---------------------------------------------------------------------- */
+/* CPU baseline timing                                                     */
+/* ---------------------------------------------------------------------- */
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static int cmp_double(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+double orc_time_cpu_depth_sort(const float* positions, size_t count, const float vp[16],
+                               float tol, int repeats, size_t* visible_out) {
+    uint32_t* keys = (uint32_t*)malloc((count ? count : 1) * sizeof(uint32_t));
+    uint32_t* idx = (uint32_t*)malloc((count ? count : 1) * sizeof(uint32_t));
+    double* t = (double*)malloc((size_t)(repeats > 0 ? repeats : 1) * sizeof(double));
+    size_t v = 0;
+    for (int w = 0; w < 1; ++w) v = orc_cpu_depth_sort(positions, count, vp, tol, keys, idx);
+    for (int i = 0; i < repeats; ++i) {
+        double t0 = now_s();
+        v = orc_cpu_depth_sort(positions, count, vp, tol, keys, idx);
+        t[i] = now_s() - t0;
+    }
+    qsort(t, (size_t)repeats, sizeof(double), cmp_double);
+    double med = repeats > 0 ? t[repeats / 2] : 0.0;
+    if (visible_out) *visible_out = v;
+    free(keys);
+    free(idx);
+    free(t);
+    return med;
+}

@@ -1,0 +1,293 @@
+"""splatter_amd -- MI355X-native splat pipeline behind splatter's Renderer/Scene API.
+
+Python mirror of the reference crate's public surface (src/lib.rs:1-3):
+
+    renderer::DepthSorting      -> DepthSorting
+    renderer::Configuration     -> Configuration
+    renderer::Renderer::new     -> Renderer(device, config)
+    Renderer::render_frame      -> Renderer.render_frame(frame, camera_matrix, view_matrix, scene)
+    scene::Scene::new           -> Scene(renderer, splat_count)
+    Scene::new_dev_test         -> Scene.new_dev_test(renderer)
+    Scene::parse_file_header    -> Scene.parse_file_header(path)
+    Scene::load_chunk           -> Scene.load_chunk(path, header_size, begin, end)
+    Scene.splat_count           -> Scene.splat_count
+
+wgpu's Device/Queue pair is a Device (one HIP device and stream) and the
+TextureView + viewport_size pair is a Frame.  Everything runs in
+libsplatter_amd.so (HIP kernels for gfx950); this module only marshals.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import SplConfiguration, SplatterError, check, lib
+
+__all__ = [
+    "DepthSorting", "FrameFormat", "BlendMode", "Configuration", "Device", "Renderer",
+    "Scene", "Frame", "SplatterError", "SERIALIZED_SPLAT_DTYPE",
+]
+
+
+class DepthSorting(enum.IntEnum):
+    """renderer.rs:14-23"""
+    NONE = 0
+    CPU = 1
+    GPU = 2
+    GPU_INDIRECT_DRAW = 3
+
+
+class FrameFormat(enum.IntEnum):
+    BGRA8_UNORM = 0
+    RGBA8_UNORM = 1
+
+
+class BlendMode(enum.IntEnum):
+    UNORM8 = 0
+    F32 = 1
+
+
+@dataclass
+class Configuration:
+    """renderer.rs:26-47 with the showcase's values (showcase.rs:39-50)."""
+    frame_format: FrameFormat = FrameFormat.BGRA8_UNORM
+    depth_sorting: DepthSorting = DepthSorting.GPU
+    use_covariance_for_scale: bool = True
+    use_unaligned_rectangles: bool = True
+    spherical_harmonics_order: int = 2
+    max_splat_count: int = 1024 * 1024 * 4
+    radix_bits_per_digit: int = 8
+    frustum_culling_tolerance: float = 1.1
+    ellipse_margin: float = 2.0
+    splat_scale: float = 1.0
+    field_of_view_y: float = math.pi * 0.5
+    blend_mode: BlendMode = BlendMode.UNORM8
+    tile_pairs_per_splat: int = 4
+
+    def to_c(self) -> SplConfiguration:
+        c = SplConfiguration()
+        c.frame_format = int(self.frame_format)
+        c.depth_sorting = int(self.depth_sorting)
+        c.use_covariance_for_scale = int(bool(self.use_covariance_for_scale))
+        c.use_unaligned_rectangles = int(bool(self.use_unaligned_rectangles))
+        c.spherical_harmonics_order = int(self.spherical_harmonics_order)
+        c.max_splat_count = int(self.max_splat_count)
+        c.radix_bits_per_digit = int(self.radix_bits_per_digit)
+        c.frustum_culling_tolerance = float(self.frustum_culling_tolerance)
+        c.ellipse_margin = float(self.ellipse_margin)
+        c.splat_scale = float(self.splat_scale)
+        c.field_of_view_y = float(np.float32(self.field_of_view_y))
+        c.blend_mode = int(self.blend_mode)
+        c.tile_pairs_per_splat = int(self.tile_pairs_per_splat)
+        return c
+
+
+# scene.rs:17-24 SerializedSplat, 248 bytes
+SERIALIZED_SPLAT_DTYPE = np.dtype([
+    ("center", "<f4", 3), ("n", "<f4", 3), ("color", "<f4", 48),
+    ("alpha", "<f4"), ("scale", "<f4", 3), ("rotation", "<f4", 4),
+])
+assert SERIALIZED_SPLAT_DTYPE.itemsize == 248
+
+
+def _mat16(m) -> C.Array:
+    a = np.ascontiguousarray(np.asarray(m, dtype=np.float32).reshape(16))
+    return (C.c_float * 16)(*a.tolist())
+
+
+class Device:
+    """wgpu Device + Queue: one HIP device with one stream."""
+
+    def __init__(self, ordinal: int = 0):
+        h = C.c_void_p()
+        check(lib().spl_device_open(int(ordinal), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self) -> None:
+        check(lib().spl_synchronize(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            lib().spl_device_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Frame:
+    """Render target of width x height RGBA8/BGRA8 pixels in device memory."""
+
+    def __init__(self, device: Device, width: int, height: int):
+        h = C.c_void_p()
+        check(lib().spl_frame_create(device.handle, int(width), int(height), C.byref(h)))
+        self._h, self.device, self.width, self.height = h, device, int(width), int(height)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def read(self) -> np.ndarray:
+        out = np.empty((self.height, self.width, 4), dtype=np.uint8)
+        check(lib().spl_frame_read(self._h, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if self._h:
+            lib().spl_frame_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Renderer:
+    """renderer.rs:64-84 Renderer."""
+
+    def __init__(self, device: Device, config: Configuration | None = None):
+        self.config = config or Configuration()
+        self.device = device
+        h = C.c_void_p()
+        cc = self.config.to_c()
+        check(lib().spl_renderer_create(device.handle, C.byref(cc), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def render_frame(self, frame: Frame, camera_matrix, view_matrix, scene: "Scene") -> None:
+        """renderer.rs:355-363: camera_matrix = motor3d_to_mat4(motor),
+        view_matrix = motor3d_to_mat4(motor.inverse()), column-major 4x4."""
+        check(lib().spl_render_frame(self._h, scene.handle, frame.handle,
+                                     _mat16(camera_matrix), _mat16(view_matrix)))
+
+    def render_views(self, frames, camera_matrices, view_matrices, scene: "Scene") -> None:
+        n = len(frames)
+        cams = np.ascontiguousarray(np.asarray(camera_matrices, dtype=np.float32).reshape(n, 16))
+        views = np.ascontiguousarray(np.asarray(view_matrices, dtype=np.float32).reshape(n, 16))
+        arr = (C.c_void_p * n)(*[f.handle.value for f in frames])
+        check(lib().spl_render_views(self._h, scene.handle, n, C.cast(arr, C.POINTER(C.c_void_p)),
+                                     cams.ctypes.data_as(C.POINTER(C.c_float)),
+                                     views.ctypes.data_as(C.POINTER(C.c_float))))
+
+    # ---- parity taps ---------------------------------------------------
+    def visible_count(self) -> int:
+        v = C.c_uint64()
+        check(lib().spl_debug_visible_count(self._h, C.byref(v)))
+        return int(v.value)
+
+    def pair_count(self) -> int:
+        v = C.c_uint64()
+        check(lib().spl_debug_pair_count(self._h, C.byref(v)))
+        return int(v.value)
+
+    def read_entries(self):
+        n = self.visible_count()
+        keys = np.empty(max(n, 1), np.uint32)
+        idx = np.empty(max(n, 1), np.uint32)
+        cnt = C.c_uint64()
+        check(lib().spl_debug_read_entries(self._h, keys.ctypes.data_as(C.c_void_p),
+                                           idx.ctypes.data_as(C.c_void_p), n, C.byref(cnt)))
+        return keys[:cnt.value], idx[:cnt.value]
+
+    def read_records(self) -> np.ndarray:
+        n = self.visible_count()
+        out = np.empty((max(n, 1), 10), np.float32)
+        cnt = C.c_uint64()
+        check(lib().spl_debug_read_records(self._h, out.ctypes.data_as(C.c_void_p), n, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def enable_conics(self, enable: bool = True) -> None:
+        check(lib().spl_debug_enable_conics(self._h, int(enable)))
+
+    def read_conics(self) -> np.ndarray:
+        n = self.visible_count()
+        out = np.empty((max(n, 1), 6), np.float64)
+        cnt = C.c_uint64()
+        check(lib().spl_debug_read_conics(self._h, out.ctypes.data_as(C.c_void_p), n, C.byref(cnt)))
+        return out[:cnt.value]
+
+    def enable_timing(self, enable: bool = True) -> None:
+        check(lib().spl_debug_enable_timing(self._h, int(enable)))
+
+    def stage_times(self) -> np.ndarray:
+        out = (C.c_float * 7)()
+        check(lib().spl_debug_stage_times(self._h, out, 7))
+        return np.array(out[:], dtype=np.float64)
+
+    def close(self) -> None:
+        if self._h:
+            lib().spl_renderer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """scene.rs:35-41 Scene: splats resident in HBM."""
+
+    def __init__(self, renderer: Renderer, splat_count: int, _handle=None):
+        self.renderer = renderer
+        if _handle is None:
+            h = C.c_void_p()
+            check(lib().spl_scene_create(renderer.handle, int(splat_count), C.byref(h)))
+            _handle = h
+        self._h = _handle
+        c = C.c_uint64()
+        check(lib().spl_scene_splat_count(self._h, C.byref(c)))
+        self.splat_count = int(c.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def new_dev_test(cls, renderer: Renderer) -> "Scene":
+        h = C.c_void_p()
+        check(lib().spl_scene_new_dev_test(renderer.handle, C.byref(h)))
+        return cls(renderer, 3, _handle=h)
+
+    @staticmethod
+    def parse_file_header(path: str):
+        hs, cnt = C.c_uint16(), C.c_uint64()
+        check(lib().spl_parse_file_header(path.encode(), C.byref(hs), C.byref(cnt)))
+        return int(hs.value), int(cnt.value)
+
+    def load_chunk(self, path: str, file_header_size: int, begin: int, end: int) -> None:
+        check(lib().spl_scene_load_chunk(self._h, path.encode(), int(file_header_size), int(begin), int(end)))
+
+    def load_records(self, first: int, records: np.ndarray) -> None:
+        rec = np.ascontiguousarray(records)
+        if rec.dtype != SERIALIZED_SPLAT_DTYPE:
+            rec = rec.view(SERIALIZED_SPLAT_DTYPE) if rec.dtype.itemsize == 248 else rec.astype(SERIALIZED_SPLAT_DTYPE)
+        check(lib().spl_scene_load_records(self._h, int(first), int(rec.shape[0]), rec.ctypes.data_as(C.c_void_p)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().spl_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,294 @@
+// project.hip -- per visible splat: spherical-harmonics colour (f32), the
+// contour conic of its ellipsoid and the rotated rectangle around it (f64).
+//
+// This is the per-splat half of the reference's vertex stage
+// (shaders.wgsl:507-547) evaluated once per visible splat instead of once per
+// quad vertex.  Entry c of the compacted list (splat compact_idx[c]) writes
+// record c: rgba | A00 A10 A01 A11 | tx ty bbox, 48 B, where A = (col0 col1)
+// and t are the rectangle's affine map (shaders.wgsl:543-547) in view-plane
+// units and bbox is a conservative pixel box of what the rectangle can cover.
+//
+// The conic and its decomposition run in FP64: the contour path's semi-axes
+// (shaders.wgsl:217-237) lose most of their digits in FP32 to the
+// cancellation in e = M.zz - t^T M t.  Products and sums follow the WGSL's
+// order with no contraction (-ffp-contract=off).
+#include "spl_device.h"
+#include "spl_internal.h"
+
+namespace spl {
+
+namespace {
+
+// WGSL column-major 3x3: m[c][r].  (X Y)[c][r] = sum_k X[k][r] Y[c][k].
+__device__ __forceinline__ void mat3_mul(const double X[3][3], const double Y[3][3], double out[3][3]) {
+  double tmp[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) tmp[c][r] = (X[0][r] * Y[c][0] + X[1][r] * Y[c][1]) + X[2][r] * Y[c][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[c][r] = tmp[c][r];
+}
+
+__device__ __forceinline__ void mat3_transpose(const double X[3][3], double out[3][3]) {
+  double tmp[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) tmp[c][r] = X[r][c];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) out[c][r] = tmp[c][r];
+}
+
+__device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+// shaders.wgsl:63-79 quatToMat
+__device__ __forceinline__ void quat_to_mat(const double p[4], double m[3][3]) {
+  const double s2 = sqrt(2.0);
+  const double q0 = p[0] * s2, q1 = p[1] * s2, q2 = p[2] * s2, q3 = p[3] * s2;
+  const double yy = q1 * q1, yz = q1 * q2, yw = q1 * q3, yx = q1 * q0;
+  const double zz = q2 * q2, zw = q2 * q3, zx = q2 * q0;
+  const double ww = q3 * q3, wx = q3 * q0;
+  m[0][0] = (1.0 - zz) - ww; m[0][1] = yz + wx; m[0][2] = yw - zx;
+  m[1][0] = yz - wx; m[1][1] = (1.0 - yy) - ww; m[1][2] = zw + yx;
+  m[2][0] = yw + zx; m[2][1] = zw - yx; m[2][2] = (1.0 - yy) - zz;
+}
+
+// shaders.wgsl:125-169 projectedContourOfEllipsoid
+__device__ void contour(const double scale[3], const double rot[4], const double pos[3], const Uniforms& u,
+                        double M[3][3]) {
+  double C[3][3], T[3][3], A[3][3], Ct[3][3], Tt[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) C[c][r] = (double)u.camera_matrix[4 * c + r];
+  quat_to_mat(rot, T);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] / scale[c];
+  double ro[3], lo[3], ls[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ro[i] = (double)u.camera_matrix[12 + i] - pos[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) lo[c] = (ro[0] * T[c][0] + ro[1] * T[c][1]) + ro[2] * T[c][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ls[i] = lo[i] * lo[i];
+  const double dg0 = (1.0 - ls[1]) - ls[2], dg1 = (1.0 - ls[0]) - ls[2], dg2 = (1.0 - ls[0]) - ls[1];
+  const double tr0 = lo[1] * lo[2], tr1 = lo[0] * lo[2], tr2 = lo[0] * lo[1];
+  A[0][0] = dg0; A[0][1] = tr2; A[0][2] = tr1;
+  A[1][0] = tr2; A[1][1] = dg1; A[1][2] = tr0;
+  A[2][0] = tr1; A[2][1] = tr0; A[2][2] = dg2;
+  mat3_transpose(C, Ct);
+  mat3_mul(Ct, T, T);
+  mat3_transpose(T, Tt);
+  mat3_mul(T, A, M);
+  mat3_mul(M, Tt, M);
+}
+
+// shaders.wgsl:99-118 projectedCovarianceOfEllipsoid
+__device__ void covariance(const double scale[3], const double rot[4], const double pos[3], const Uniforms& u,
+                           double cov[3][3]) {
+  double C[3][3], T[3][3], J[3][3], Tt[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) C[c][r] = (double)u.camera_matrix[4 * c + r];
+  quat_to_mat(rot, T);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] * scale[c];
+  const float* V = u.view_matrix;
+  double vp[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    vp[r] = (((double)V[r] * pos[0] + (double)V[4 + r] * pos[1]) + (double)V[8 + r] * pos[2]) +
+            (double)V[12 + r] * 1.0;
+  double qx = vp[0] / vp[2], qy = vp[1] / vp[2];
+  qx = qx < -1.0 ? -1.0 : (qx > 1.0 ? 1.0 : qx);
+  qy = qy < -1.0 ? -1.0 : (qy > 1.0 ? 1.0 : qy);
+  vp[0] = qx * vp[2];
+  vp[1] = qy * vp[2];
+  J[0][0] = 1.0 / vp[2]; J[0][1] = 0.0; J[0][2] = -vp[0] / (vp[2] * vp[2]);
+  J[1][0] = 0.0; J[1][1] = 1.0 / vp[2]; J[1][2] = -vp[1] / (vp[2] * vp[2]);
+  J[2][0] = 0.0; J[2][1] = 0.0; J[2][2] = 0.0;
+  mat3_transpose(T, Tt);
+  mat3_mul(Tt, C, Tt);
+  mat3_mul(Tt, J, Tt);
+  double Ttt[3][3];
+  mat3_transpose(Tt, Ttt);
+  mat3_mul(Ttt, Tt, cov);
+}
+
+// shaders.wgsl:248-294 (f32; same constants and evaluation order)
+__constant__ float kShc[16] = {
+    0.28209479177387814f, -0.4886025119029199f, 0.4886025119029199f, -0.4886025119029199f,
+    1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f, -1.0925484305920792f,
+    0.5462742152960396f, -0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+    0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f, -0.5900435899266435f,
+};
+
+template <int L>
+__device__ __forceinline__ void sh_color(const float d[3], const float* sh, float out[3]) {
+  const float x = d[0], y = d[1], z = d[2];
+  const float xx = x * x, yy = y * y, zz = z * z;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float col = 0.5f;
+    col = col + kShc[0] * sh[c];
+    if (L > 0) {
+      col = col + (kShc[1] * sh[3 + c]) * y;
+      col = col + (kShc[2] * sh[6 + c]) * z;
+      col = col + (kShc[3] * sh[9 + c]) * x;
+    }
+    if (L > 1) {
+      col = col + ((kShc[4] * sh[12 + c]) * x) * y;
+      col = col + ((kShc[5] * sh[15 + c]) * y) * z;
+      col = col + ((kShc[6] * sh[18 + c]) * ((2.0f * zz - xx) - yy));
+      col = col + ((kShc[7] * sh[21 + c]) * x) * z;
+      col = col + (kShc[8] * sh[24 + c]) * (xx - yy);
+    }
+    if (L > 2) {
+      col = col + ((kShc[9] * sh[27 + c]) * y) * (3.0f * xx - yy);
+      col = col + (((kShc[10] * sh[30 + c]) * x) * y) * z;
+      col = col + ((kShc[11] * sh[33 + c]) * y) * ((4.0f * zz - xx) - yy);
+      col = col + ((kShc[12] * sh[36 + c]) * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+      col = col + ((kShc[13] * sh[39 + c]) * x) * ((4.0f * zz - xx) - yy);
+      col = col + ((kShc[14] * sh[42 + c]) * z) * (xx - yy);
+      col = col + ((kShc[15] * sh[45 + c]) * x) * (xx - 3.0f * yy);
+    }
+    out[c] = col;
+  }
+}
+
+__device__ __forceinline__ bool finite(float x) { return (__float_as_uint(x) & 0x7F800000u) != 0x7F800000u; }
+
+}  // namespace
+
+template <int L>
+__global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
+  clear_words(a.clear_a, a.clear_a_words);
+  clear_words(a.clear_b, a.clear_b_words);
+  const uint32_t n = *a.count;
+  const Uniforms& u = a.u;
+  constexpr int SH_FLOATS = 3 * (L + 1) * (L + 1);
+  constexpr int SH_F4 = (SH_FLOATS + 3) / 4;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    const uint32_t i = a.compact_idx[c];
+    const float cx = a.scene.centers[3 * (size_t)i + 0];
+    const float cy = a.scene.centers[3 * (size_t)i + 1];
+    const float cz = a.scene.centers[3 * (size_t)i + 2];
+    const float4 rq = a.scene.rotations[i];
+    const float4 sa = a.scene.scale_alpha[i];
+    float sh[SH_F4 * 4];
+    const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
+#pragma unroll
+    for (int k = 0; k < SH_F4; ++k) {
+      const float4 v = shp[k];
+      sh[4 * k + 0] = v.x; sh[4 * k + 1] = v.y; sh[4 * k + 2] = v.z; sh[4 * k + 3] = v.w;
+    }
+
+    // shaders.wgsl:531-532: colour from the normalised ray direction
+    float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
+    const float len = sqrtf((rd[0] * rd[0] + rd[1] * rd[1]) + rd[2] * rd[2]);
+    rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
+    float rgb[3];
+    sh_color<L>(rd, sh, rgb);
+
+    // shaders.wgsl:533-547
+    const double scale[3] = {(double)sa.x * (double)u.splat_scale, (double)sa.y * (double)u.splat_scale,
+                             (double)sa.z * (double)u.splat_scale};
+    const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
+    const double pos[3] = {(double)cx, (double)cy, (double)cz};
+    double M[3][3];
+    contour(scale, rot, pos, u, M);
+    // extractTranslationOfEllipse (shaders.wgsl:179-191)
+    const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
+    const double inv_disc = 1.0 / disc;
+    const double t0 = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * inv_disc;
+    const double t1 = (M[0][1] * M[0][2] - M[0][0] * M[1][2]) * inv_disc;
+    // extractRotationOfEllipse (shaders.wgsl:193-215)
+    const double dxy = M[0][0] - M[1][1];
+    const double ra = dxy * dxy;
+    const double rb = ra + (4.0 * M[0][1]) * M[0][1];
+    const double rc = 0.5 * sqrt(ra / rb);
+    double j = sqrt(0.5 - rc);
+    double k = (-sqrt(0.5 + rc) * sgn(M[0][1])) * sgn(dxy);
+    if (M[0][1] < 0.0 || dxy < 0.0) { k = -k; j = -j; }
+    if (dxy < 0.0) { const double tt = j; j = -k; k = tt; }
+    double s0, s1;
+    if (a.use_covariance_for_scale) {
+      // extractScaleOfCovariance (shaders.wgsl:240-246)
+      double cov[3][3];
+      covariance(scale, rot, pos, u, cov);
+      const double cd = cov[0][0] - cov[1][1];
+      const double b = sqrt(cd * cd + (4.0 * cov[0][1]) * cov[0][1]);
+      s0 = sqrt(((cov[0][0] + cov[1][1]) + b) * 0.5);
+      s1 = sqrt(((cov[0][0] + cov[1][1]) - b) * 0.5);
+    } else {
+      // extractScaleOfEllipse (shaders.wgsl:217-237)
+      const double d = ((2.0 * M[0][1]) * j) * k;
+      const double e = M[2][2] - (((M[0][0] * t0) * t0 + (M[1][1] * t1) * t1) + ((2.0 * M[0][1]) * t0) * t1);
+      s0 = sqrt(fabs(e / (((M[0][0] * k) * k + (M[1][1] * j) * j) - d)));
+      s1 = sqrt(fabs(e / (((M[0][0] * j) * j + (M[1][1] * k) * k) + d)));
+    }
+    const double bias = (double)u.ellipse_size_bias;
+    const double sa0 = bias + s0, sb0 = bias + s1;
+    const float A00 = (float)(k * sa0), A10 = (float)(-j * sa0);
+    const float A01 = (float)(j * sb0), A11 = (float)(k * sb0);
+    const float tx = (float)t0, ty = (float)t1;
+
+    // conservative pixel box of the covered region
+    uint32_t box_lo = 0xFFFFFFFFu, box_hi = 0u;  // empty: x0 > x1
+    const bool ok = finite(A00) && finite(A10) && finite(A01) && finite(A11) && finite(tx) && finite(ty);
+    if (ok) {
+      // in f64 so that a far-off centre with a huge extent cannot cancel away
+      const double m = (double)u.ellipse_margin;
+      double ex, ey;
+      if (a.use_unaligned_rectangles) {
+        ex = (fabs((double)A00) + fabs((double)A01)) * m;
+        ey = (fabs((double)A10) + fabs((double)A11)) * m;
+      } else {
+        ex = sqrt(fmax((double)A00 * A00 + (double)A10 * A10, (double)A01 * A01 + (double)A11 * A11)) * m;
+        ey = ex;
+      }
+      const double W = (double)u.image_size[0], H = (double)u.image_size[1];
+      const double vw = (double)u.view_size[0], vh = (double)u.view_size[1];
+      const double pcx = ((double)tx / vw + 1.0) * 0.5 * W, pcy = (1.0 - (double)ty / vh) * 0.5 * H;
+      const double rx = ex / vw * 0.5 * W + 2.0, ry = ey / vh * 0.5 * H + 2.0;
+      const double fx0 = floor(pcx - rx), fx1 = ceil(pcx + rx);
+      const double fy0 = floor(pcy - ry), fy1 = ceil(pcy + ry);
+      if (fx1 >= 0.0 && fy1 >= 0.0 && fx0 <= W - 1.0 && fy0 <= H - 1.0) {
+        const uint32_t x0 = (uint32_t)fmax(fx0, 0.0), y0 = (uint32_t)fmax(fy0, 0.0);
+        const uint32_t x1 = (uint32_t)fmin(fx1, W - 1.0), y1 = (uint32_t)fmin(fy1, H - 1.0);
+        box_lo = x0 | (y0 << 16);
+        box_hi = x1 | (y1 << 16);
+      }
+    }
+    float4* rec = a.records + 3 * (size_t)c;
+    rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
+    rec[1] = make_float4(A00, A10, A01, A11);
+    rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
+    if (a.conics) {
+      double* cn = a.conics + 6 * (size_t)c;
+      cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];
+      cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
+    }
+  }
+}
+
+void launch_project(const ProjectLaunch& a, hipStream_t s) {
+  switch (a.sh_order) {
+    case 0: hipLaunchKernelGGL((k_project<0>), dim3(a.grid), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_project<1>), dim3(a.grid), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_project<2>), dim3(a.grid), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_project<3>), dim3(a.grid), dim3(256), 0, s, a); break;
+  }
+}
+
+}  // namespace spl

@@ -1,0 +1,276 @@
+// raster.hip -- compute rasteriser for the projected rectangles.
+//
+// Replaces the fixed-function half of the reference's render pass
+// (renderer.rs:247-291,435-455) and its fragment stage (shaders.wgsl:580-592):
+//  bin     one (screen tile, record) pair per 16x16 tile a splat's pixel box
+//          touches, written in draw order (single-pass scan with decoupled
+//          look-back, pairs emitted by a balanced block-wide expansion), plus
+//          the digit histograms of the tile ids for the stable pair sort.
+//  ranges  [start, end) of each tile's run in the tile-sorted pair list.
+//  raster  one 256-lane block per tile, one lane per pixel: records staged
+//          in LDS 256 at a time, blended front to back in draw order with the
+//          reference's blend state (colour = src*DstAlpha + dst, alpha =
+//          dst*(1-srcA), cleared to (0,0,0,1)), the unorm8 target emulated by
+//          clamping the source and re-quantising after every blend; a tile
+//          stops once every pixel's transmittance is 0.
+#include "spl_device.h"
+#include "spl_internal.h"
+
+namespace spl {
+
+namespace {
+
+// exp(x): Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, degree-7 Taylor
+// polynomial of e^r in Horner form, 2^k through the exponent field.  Only
+// IEEE add/mul/floor, so it rounds the same on the CPU and the GPU.
+__device__ __forceinline__ float spl_exp(float x) {
+  if (!(x > -87.0f)) return 0.0f;
+  if (x > 88.0f) return __int_as_float(0x7F800000);
+  const float kf = floorf(x * 1.44269502f + 0.5f);
+  const float r = (x - kf * 0.693145752f) - kf * 1.42860677e-06f;
+  float p = 0.000198412701f;
+  p = p * r + 0.00138888892f;
+  p = p * r + 0.00833333377f;
+  p = p * r + 0.0416666679f;
+  p = p * r + 0.166666672f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return p * __int_as_float((int)(kf + 127.0f) << 23);
+}
+
+__device__ __forceinline__ float q8(float x) {
+  x = fminf(fmaxf(x, 0.0f), 1.0f);
+  return floorf(x * 255.0f + 0.5f);
+}
+
+__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+}  // namespace
+
+template <int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
+  constexpr int TILE = THREADS * ITEMS;
+  constexpr int WAVES = THREADS / kWave;
+  __shared__ uint32_t s_off[TILE + 1];
+  __shared__ uint32_t s_org[TILE];
+  __shared__ uint32_t s_w[TILE];
+  __shared__ uint32_t s_rec[TILE];
+  __shared__ uint32_t s_hist[kMaxPairPasses * 256];
+  __shared__ uint32_t s_tmp[WAVES];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+
+  const uint32_t tid = threadIdx.x;
+  for (int i = tid; i < a.pair_passes * 256; i += THREADS) s_hist[i] = 0u;
+  const uint32_t n = *a.count;
+  const uint32_t num_tiles = (n + TILE - 1) / TILE;
+  if (num_tiles == 0 && blockIdx.x == 0 && tid == 0) {
+    a.counters[kPairs] = 0u;
+    a.counters[kPairOverflow] = 0u;
+  }
+  const uint32_t tiles_x = a.tiles_x;
+
+  while (true) {
+    if (tid == 0) s_tile = atomicAdd(&a.counters[kCtrBin], 1u);
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= num_tiles) break;
+    uint32_t cnt[ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t slot = tid * ITEMS + k;
+      const uint32_t r = t * TILE + slot;
+      cnt[k] = 0;
+      if (r < n) {
+        const uint32_t c = a.order ? a.order[r] : r;
+        const float4 q = a.records[3 * (size_t)c + 2];
+        const uint32_t lo = __float_as_uint(q.z), hi = __float_as_uint(q.w);
+        const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;
+        const uint32_t x1 = (hi & 0xFFFFu) / kTileSize, y1 = (hi >> 16) / kTileSize;
+        if (lo != 0xFFFFFFFFu && x1 >= x0 && y1 >= y0) {
+          cnt[k] = (x1 - x0 + 1) * (y1 - y0 + 1);
+          s_org[slot] = x0 | (y0 << 16);
+          s_w[slot] = x1 - x0 + 1;
+        }
+        s_rec[slot] = c;
+      }
+      sum += cnt[k];
+    }
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      s_off[tid * ITEMS + k] = excl;
+      excl += cnt[k];
+    }
+    if (tid == 0) {
+      s_off[TILE] = total;
+      const uint64_t prefix = lookback64(a.status, t, (uint64_t)total);
+      s_prefix = prefix;
+      if (t == num_tiles - 1) {
+        const uint64_t k_all = prefix + total;
+        a.counters[kPairs] = (uint32_t)(k_all < a.pair_cap ? k_all : a.pair_cap);
+        a.counters[kPairOverflow] = k_all > a.pair_cap ? 1u : 0u;
+      }
+    }
+    __syncthreads();
+    const uint64_t prefix = s_prefix;
+    for (uint32_t s = tid; s < total; s += THREADS) {
+      uint32_t lo = 0, hi = TILE - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_off[mid] <= s) lo = mid; else hi = mid - 1;
+      }
+      const uint64_t pos = prefix + s;
+      if (pos < a.pair_cap) {
+        const uint32_t k = s - s_off[lo];
+        const uint32_t w = s_w[lo], org = s_org[lo];
+        const uint32_t tile = ((org >> 16) + k / w) * tiles_x + (org & 0xFFFFu) + k % w;
+        a.pair_keys[pos] = tile;
+        a.pair_vals[pos] = s_rec[lo];
+        for (int q = 0; q < a.pair_passes; ++q) atomicAdd(&s_hist[q * 256 + ((tile >> (8 * q)) & 255u)], 1u);
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < a.pair_passes * 256; i += THREADS)
+    if (s_hist[i]) atomicAdd(&a.hist[i], s_hist[i]);
+  finish_persistent(&a.counters[kCtrBin], &a.counters[kDone + kCtrBin]);
+}
+
+__global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
+  clear_words(a.clear_a, a.clear_a_words);
+  clear_words(a.clear_b, a.clear_b_words);
+  uint32_t n = a.counters[kPairs];
+  if (n > a.pair_cap) n = a.pair_cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t t = a.pair_keys[i];
+    if (t >= a.tiles) continue;
+    if (i == 0 || a.pair_keys[i - 1] != t) a.ranges[t].x = i;
+    if (i == n - 1 || a.pair_keys[i + 1] != t) a.ranges[t].y = i + 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
+  constexpr int B = 256;
+  __shared__ float s_i00[B], s_i01[B], s_i10[B], s_i11[B], s_tx[B], s_ty[B], s_half[B];
+  __shared__ float s_r[B], s_g[B], s_b[B], s_a[B];
+  __shared__ uint32_t s_lo[B], s_hi[B];
+
+  const Uniforms& u = a.u;
+  const uint32_t T = u.tiles_x * u.tiles_y;
+  // XCD-aware order: blocks are dealt round-robin to the 8 XCDs, so give
+  // each XCD a contiguous band of tiles (neighbouring tiles share records).
+  const uint32_t per_xcd = gridDim.x / 8;
+  const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+  if (tile >= T) return;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t px = (tile % u.tiles_x) * kTileSize + (tid & 15u);
+  const uint32_t py = (tile / u.tiles_x) * kTileSize + (tid >> 4);
+  const uint32_t W = u.image_size[0], H = u.image_size[1];
+  const bool inside = px < W && py < H;
+  const uint2 range = a.ranges[tile];
+  __syncthreads();
+  if (tid == 0) a.ranges[tile] = make_uint2(0u, 0u);
+
+  const float qx = ((((float)px + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
+  const float qy = (1.0f - (((float)py + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
+  const uint32_t row0 = (tile / u.tiles_x) * kTileSize + (tid >> 6) * 4, row1 = row0 + 3;
+  const bool f32 = a.blend_f32 != 0;
+  const bool unaligned = a.use_unaligned_rectangles != 0;
+  const float margin = u.ellipse_margin;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = f32 ? 1.0f : 255.0f;
+  bool done = !inside;
+
+  const uint32_t visible = *a.visible;
+  for (uint32_t base = range.x; base < range.y; base += B) {
+    const uint32_t cnt = (range.y - base) < (uint32_t)B ? (range.y - base) : (uint32_t)B;
+    if (tid < cnt) {
+      uint32_t c = a.pair_vals[base + tid];
+      if (c >= visible) c = 0;  // only after an upstream error; never read past the records
+      const float4 r0 = a.records[3 * (size_t)c + 0];
+      const float4 r1 = a.records[3 * (size_t)c + 1];
+      const float4 r2 = a.records[3 * (size_t)c + 2];
+      const float A00 = r1.x, A10 = r1.y, A01 = r1.z, A11 = r1.w;
+      const float det = A00 * A11 - A10 * A01;
+      const float rdet = 1.0f / det;
+      s_i00[tid] = A11 * rdet;
+      s_i10[tid] = -A10 * rdet;
+      s_i01[tid] = -A01 * rdet;
+      s_i11[tid] = A00 * rdet;
+      s_tx[tid] = r2.x;
+      s_ty[tid] = r2.y;
+      s_half[tid] = unaligned ? margin : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
+      s_r[tid] = r0.x; s_g[tid] = r0.y; s_b[tid] = r0.z; s_a[tid] = r0.w;
+      s_lo[tid] = __float_as_uint(r2.z);
+      s_hi[tid] = __float_as_uint(r2.w);
+    }
+    __syncthreads();
+    if (!done) {
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t lo = s_lo[j], hi = s_hi[j];
+        const uint32_t y0 = lo >> 16, y1 = hi >> 16;
+        if (row1 < y0 || row0 > y1) continue;  // wave-uniform
+        const uint32_t x0 = lo & 0xFFFFu, x1 = hi & 0xFFFFu;
+        if (px < x0 || px > x1 || py < y0 || py > y1 || tr == 0.0f) continue;
+        const float dx = qx - s_tx[j], dy = qy - s_ty[j];
+        const float tcx = s_i00[j] * dx + s_i01[j] * dy;
+        const float tcy = s_i10[j] * dx + s_i11[j] * dy;
+        const float h = s_half[j];
+        const bool in = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
+                                  : (dx >= -h && dx < h && dy >= -h && dy < h);
+        if (!in) continue;
+        const float power = tcx * tcx + tcy * tcy;
+        const float alpha = s_a[j] * spl_exp(-0.5f * power);
+        if (!(alpha >= 1.0f / 255.0f)) continue;
+        const float sr = s_r[j] * alpha, sg = s_g[j] * alpha, sb = s_b[j] * alpha;
+        if (!f32) {
+          const float da = tr / 255.0f;
+          const float sa = clamp01(alpha);
+          c0 = q8(clamp01(sr) * da + c0 / 255.0f);
+          c1 = q8(clamp01(sg) * da + c1 / 255.0f);
+          c2 = q8(clamp01(sb) * da + c2 / 255.0f);
+          tr = q8(da * (1.0f - sa));
+        } else {
+          const float da = tr;
+          c0 = sr * da + c0;
+          c1 = sg * da + c1;
+          c2 = sb * da + c2;
+          tr = da * (1.0f - alpha);
+        }
+      }
+      done = tr == 0.0f;
+    }
+    if (__syncthreads_and(done)) break;
+  }
+  if (inside) {
+    uint32_t r, g, b, al;
+    if (!f32) {
+      r = (uint32_t)c0; g = (uint32_t)c1; b = (uint32_t)c2; al = (uint32_t)tr;
+    } else {
+      r = (uint32_t)q8(c0); g = (uint32_t)q8(c1); b = (uint32_t)q8(c2); al = (uint32_t)q8(tr);
+    }
+    a.pixels[(size_t)py * W + px] = a.bgra ? (b | (g << 8) | (r << 16) | (al << 24))
+                                            : (r | (g << 8) | (b << 16) | (al << 24));
+  }
+}
+
+void launch_bin(const BinLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_bin<kBinThreads, kBinItems>), dim3(a.grid), dim3(kBinThreads), 0, s, a);
+}
+
+void launch_ranges(const RangesLaunch& a, uint32_t max_pairs, hipStream_t s) {
+  uint32_t grid = (max_pairs + 255) / 256;
+  if (grid > 4096) grid = 4096;
+  if (grid == 0) grid = 1;
+  hipLaunchKernelGGL(k_tile_ranges, dim3(grid), dim3(256), 0, s, a);
+}
+
+void launch_raster(const RasterLaunch& a, hipStream_t s) {
+  const uint32_t T = a.u.tiles_x * a.u.tiles_y;
+  const uint32_t grid = ((T + 7) / 8) * 8;
+  hipLaunchKernelGGL(k_raster, dim3(grid), dim3(256), 0, s, a);
+}
+
+}  // namespace spl

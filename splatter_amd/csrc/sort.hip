@@ -1,0 +1,196 @@
+// sort.hip -- one LSD digit pass of a onesweep radix sort over (key, value)
+// pairs, replacing radixSortB/radixSortC (shaders.wgsl:338-498).
+//
+// Differences from the reference's pass, all for gfx950:
+//  - ranking is warp-level multi-split (WLMS) on wave64 ballots: BITS
+//    ballots give each lane the mask of lanes holding the same digit, mbcnt
+//    its rank among them, and a per-wave LDS counter (read by all peers,
+//    bumped by the lowest peer) carries the rank across the wave's items.
+//    Items are visited in input order, so the pass is stable -- the
+//    reference ranks with LDS atomicAdd (shaders.wgsl:428-429), which is not.
+//  - tiles are TILE = THREADS*ITEMS keys (4096) taken in order from an
+//    atomic counter by a persistent grid (reference: 1024-key tiles, one
+//    workgroup each, shaders.wgsl:406-420).
+//  - the global digit offsets come from the histogram gathered by the
+//    producer kernel and are scanned by every block on entry (radixSortB's
+//    serial scan disappears).
+//  - decoupled look-back per digit with agent-scope status words
+//    (aggregate / inclusive flags + 30-bit count, as shaders.wgsl:438-457).
+//  - keys, then values, are reordered through LDS so that global stores of
+//    one digit run are contiguous (shaders.wgsl:463-497 does the same).
+#include "spl_device.h"
+#include "spl_internal.h"
+
+namespace spl {
+
+template <int BITS, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
+  constexpr int RADIX = 1 << BITS;
+  constexpr int WAVES = THREADS / kWave;
+  constexpr int TILE = THREADS * ITEMS;
+  constexpr int DPT = (RADIX + THREADS - 1) / THREADS;  // digits per thread
+  constexpr uint32_t MASK = RADIX - 1;
+  __shared__ uint32_t s_buf[TILE];
+  __shared__ uint32_t s_wave_hist[WAVES * RADIX];
+  __shared__ uint32_t s_digit_start[RADIX];
+  __shared__ uint32_t s_local[RADIX];
+  __shared__ uint32_t s_base[RADIX];
+  __shared__ uint32_t s_tmp[WAVES];
+  __shared__ uint32_t s_tile;
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & (kWave - 1), w = tid / kWave;
+  clear_words(a.clear_a, a.clear_a_words);
+  clear_words(a.clear_b, a.clear_b_words);
+
+  uint32_t n = *a.count;
+  if (n > a.count_cap) n = a.count_cap;
+  const uint32_t num_tiles = (n + TILE - 1) / TILE;
+  const int shift = a.shift;
+
+  {  // global exclusive digit starts
+    uint32_t v[DPT], sum = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const uint32_t d = tid * DPT + k;
+      v[k] = d < (uint32_t)RADIX ? a.hist[d] : 0u;
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const uint32_t d = tid * DPT + k;
+      if (d < (uint32_t)RADIX) s_digit_start[d] = excl;
+      excl += v[k];
+    }
+  }
+
+  while (true) {
+    if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    for (int i = tid; i < WAVES * RADIX; i += THREADS) s_wave_hist[i] = 0u;
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= num_tiles) break;
+    const uint32_t tile_base = t * TILE;
+    const uint32_t tile_n = (n - tile_base) < (uint32_t)TILE ? (n - tile_base) : (uint32_t)TILE;
+
+    uint32_t key[ITEMS], val[ITEMS], pos[ITEMS];
+    const uint32_t wave_base = tile_base + w * (kWave * ITEMS) + lane;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t idx = wave_base + j * kWave;
+      const bool valid = idx < n;
+      key[j] = valid ? a.keys_in[idx] : 0xFFFFFFFFu;
+      val[j] = valid ? (a.vals_in ? a.vals_in[idx] : idx) : 0u;
+    }
+
+    // WLMS ranking, items in input order
+    volatile uint32_t* wh = s_wave_hist + w * RADIX;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const bool valid = (wave_base + j * kWave) < n;
+      const uint32_t d = (key[j] >> shift) & MASK;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < BITS; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      const uint32_t below = lanes_below(peers);
+      const uint32_t cnt = (uint32_t)__popcll(peers);
+      uint32_t base = 0;
+      if (valid) base = wh[d];
+      __builtin_amdgcn_wave_barrier();
+      if (valid && below == 0) wh[d] = base + cnt;
+      __builtin_amdgcn_wave_barrier();
+      pos[j] = base + below;
+    }
+    __syncthreads();
+
+    // per digit: exclusive prefix over waves, tile count
+    uint32_t cnt_d[DPT], sum = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const uint32_t d = tid * DPT + k;
+      uint32_t run = 0;
+      if (d < (uint32_t)RADIX) {
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ++ww) {
+          const uint32_t x = s_wave_hist[ww * RADIX + d];
+          s_wave_hist[ww * RADIX + d] = run;
+          run += x;
+        }
+      }
+      cnt_d[k] = run;
+      sum += run;
+    }
+    // publish aggregates early so later tiles can move on
+    if (t > 0) {
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const uint32_t d = tid * DPT + k;
+        if (d < (uint32_t)RADIX) store_status(a.status + (size_t)t * RADIX + d, kFlagAggregate | cnt_d[k]);
+      }
+    }
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+      const uint32_t d = tid * DPT + k;
+      if (d < (uint32_t)RADIX) {
+        const uint32_t prefix = lookback(a.status + d, t, RADIX, cnt_d[k], false);
+        s_local[d] = excl;
+        s_base[d] = s_digit_start[d] + prefix - excl;
+      }
+      excl += cnt_d[k];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      if (wave_base + j * kWave < n) {
+        const uint32_t d = (key[j] >> shift) & MASK;
+        pos[j] += s_local[d] + s_wave_hist[w * RADIX + d];
+        if (pos[j] < (uint32_t)TILE) s_buf[pos[j]] = key[j];
+      }
+    }
+    __syncthreads();
+    uint32_t out[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t i = k * THREADS + tid;
+      out[k] = 0xFFFFFFFFu;
+      if (i < tile_n) {
+        const uint32_t kk = s_buf[i];
+        out[k] = s_base[(kk >> shift) & MASK] + i;
+        // a position past n means the histogram and the keys disagree:
+        // report it instead of storing out of bounds
+        if (out[k] < n) a.keys_out[out[k]] = kk;
+        else *a.error_flag = 1u;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+      if (wave_base + j * kWave < n && pos[j] < (uint32_t)TILE) s_buf[pos[j]] = val[j];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t i = k * THREADS + tid;
+      if (i < tile_n && out[k] < n) a.vals_out[out[k]] = s_buf[i];
+    }
+    __syncthreads();
+  }
+  finish_persistent(a.tile_counter, a.done_counter);
+}
+
+void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s) {
+  if (bits == 10)
+    hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreads, kSortItems>), dim3(a.grid), dim3(kSortThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreads, kSortItems>), dim3(a.grid), dim3(kSortThreads), 0, s, a);
+}
+
+}  // namespace spl

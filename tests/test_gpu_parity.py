@@ -1,0 +1,160 @@
+"""GPU parity of the HIP pipeline against the CPU oracle, through the C ABI.
+
+Bars (stated here, checked below):
+  visible set + draw order   bit-exact vs the CPU depth-sort fallback
+                             (renderer.rs:372-396): keys and splat indices
+  contour conic              <= 1e-5 normwise relative error vs the FP64 oracle
+  rectangle record           <= 1e-5 normwise relative error vs the FP64 oracle
+  image                      <= 2/255 max per-channel error vs the oracle raster
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import splatter_amd as sa
+from gpu_common import Case, config, device, image_diff, normwise_rel, say
+from splatter_amd.synth import Workload, look_at, orbit_poses, sphere_pose
+
+pytestmark = pytest.mark.gpu
+
+CONIC_TOL = 1e-5
+RECORD_TOL = 1e-5
+IMAGE_TOL = 2
+
+
+def check_case(case: Case, conics: bool = True, image: bool = True):
+    if conics:
+        case.renderer.enable_conics(True)
+    img = case.render()
+    keys, idx = case.renderer.read_entries()
+    okeys, oidx, orec, oimg = case.oracle(with_image=image)
+    say(f"  visible gpu={len(idx)} oracle={len(oidx)}")
+    assert np.array_equal(idx, oidx)
+    if okeys is not None:
+        assert np.array_equal(keys, okeys)
+    rec = case.renderer.read_records()
+    assert rec.shape == orec.shape
+    if len(rec):
+        assert np.array_equal(rec[:, :4], orec[:, :4])  # colour: same f32 ops
+        fin = np.all(np.isfinite(orec[:, 4:]), axis=1)
+        assert np.array_equal(fin, np.all(np.isfinite(rec[:, 4:]), axis=1))
+        assert normwise_rel(rec[fin, 4:], orec[fin, 4:]).max() <= RECORD_TOL
+    if conics and len(idx):
+        con = case.renderer.read_conics()
+        sub = np.linspace(0, len(idx) - 1, min(len(idx), 2000)).astype(np.int64)
+        _, ocon = ob.project(case.splats[oidx[sub]], case.u, case.cfg.spherical_harmonics_order,
+                             case.cfg.use_covariance_for_scale)
+        assert normwise_rel(con[sub], ocon).max() <= CONIC_TOL
+    if image:
+        d = image_diff(img, oimg)
+        say(f"  image max diff {d}/255, exact fraction {np.mean(img == oimg):.6f}")
+        assert d <= IMAGE_TOL
+    return img
+
+
+def test_dev_test_scene():
+    r = sa.Renderer(device(), config(max_splat_count=16))
+    s = sa.Scene.new_dev_test(r)
+    f = sa.Frame(device(), 320, 240)
+    cam, view = look_at((0, 0, -5), (0, 0, 0))
+    r.render_frame(f, cam, view, s)
+    keys, idx = r.read_entries()
+    assert list(idx) == [0, 1, 2]
+    assert keys[0] == keys[1] < keys[2]
+    u = ob.uniforms(cam, view, 320, 240)
+    okeys, oidx, orec, oimg = ob.render_frame(ob.dev_test_scene(), u, 3, True)
+    assert np.array_equal(keys, okeys)
+    img = f.read()[..., [2, 1, 0, 3]]
+    assert image_diff(img, oimg) <= IMAGE_TOL
+
+
+def test_gaussian_100k_720p_fov60():
+    """100k Gaussians, camera at distance 5 looking at the origin, 60 deg FOV."""
+    case = Case(Workload("gaussian", 100_000, seed=0), config(field_of_view_y=math.radians(60)),
+                1280, 720, look_at((0, 0, -5), (0, 0, 0)))
+    check_case(case)
+
+
+def test_gaussian_1m_1080p_random_pose():
+    """1M Gaussians, 1920x1080, pose on the radius-6 sphere."""
+    case = Case(Workload("gaussian", 1_000_000, seed=1), config(), 1920, 1080, sphere_pose(1))
+    check_case(case)
+
+
+def test_tie_heavy_tight_clusters():
+    """Dense clusters: many equal f32 depths -> ties broken by splat index."""
+    case = Case(Workload("tight", 300_000, seed=2), config(), 1280, 720, orbit_poses(8, radius=8)[3])
+    img = check_case(case, conics=False)
+    keys, _ = case.renderer.read_entries()
+    assert np.count_nonzero(np.diff(keys.astype(np.int64)) == 0) > 1000
+
+
+@pytest.mark.parametrize("sh,cov,unaligned,blend,fmt", [
+    (0, True, True, sa.BlendMode.UNORM8, sa.FrameFormat.RGBA8_UNORM),
+    (1, False, True, sa.BlendMode.UNORM8, sa.FrameFormat.BGRA8_UNORM),
+    (2, True, False, sa.BlendMode.F32, sa.FrameFormat.BGRA8_UNORM),
+    (3, False, False, sa.BlendMode.F32, sa.FrameFormat.RGBA8_UNORM),
+])
+def test_configuration_variants(sh, cov, unaligned, blend, fmt):
+    case = Case(Workload("gaussian", 60_000, seed=3),
+                config(spherical_harmonics_order=sh, use_covariance_for_scale=cov,
+                       use_unaligned_rectangles=unaligned, blend_mode=blend, frame_format=fmt),
+                640, 480, sphere_pose(7, radius=5))
+    check_case(case)
+
+
+def test_depth_sorting_none_draws_in_index_order():
+    case = Case(Workload("gaussian", 50_000, seed=4), config(depth_sorting=sa.DepthSorting.NONE),
+                640, 360, sphere_pose(2))
+    check_case(case, conics=False)
+
+
+def test_radix_10_bit_digits():
+    case = Case(Workload("gaussian", 200_000, seed=5), config(radix_bits_per_digit=10), 800, 600,
+                sphere_pose(3))
+    check_case(case, conics=False)
+
+
+def test_edge_cases_empty_culled_tiny_ragged():
+    # empty scene
+    case = Case(Workload("gaussian", 0, seed=0), config(max_splat_count=1), 64, 48, sphere_pose(0))
+    img = case.render()
+    assert case.renderer.visible_count() == 0
+    assert (img[..., :3] == 0).all() and (img[..., 3] == 255).all()
+    # everything behind the camera
+    cam, view = look_at((0, 0, 30), (0, 0, 60))
+    case = Case(Workload("gaussian", 5000, seed=1), config(), 64, 48, (cam, view))
+    img = case.render()
+    assert case.renderer.visible_count() == 0 and (img[..., 3] == 255).all()
+    # one splat, and ragged sizes around the 4096-entry tiles and 16-px tiles
+    for n, (w, h) in ((1, (17, 9)), (4095, (100, 37)), (4097, (33, 65)), (8191, (255, 1))):
+        case = Case(Workload("gaussian", n, seed=n), config(), w, h, sphere_pose(n, radius=4))
+        check_case(case, conics=False)
+
+
+def test_deterministic_and_frame_size_changes():
+    w = Workload("clustered", 400_000, seed=6)
+    case = Case(w, config(), 1920, 1080, orbit_poses(16)[5])
+    a = case.render()
+    ka, ia = case.renderer.read_entries()
+    small = sa.Frame(device(), 320, 200)
+    big = sa.Frame(device(), 3840, 2160)
+    case.renderer.render_frame(big, case.cam, case.view, case.scene)
+    case.renderer.render_frame(small, case.cam, case.view, case.scene)
+    b = case.render()
+    kb, ib = case.renderer.read_entries()
+    assert np.array_equal(a, b) and np.array_equal(ka, kb) and np.array_equal(ia, ib)
+
+
+def test_render_views_equals_single_frames():
+    w = Workload("clustered", 300_000, seed=8)
+    case = Case(w, config(), 640, 360, orbit_poses(4)[0])
+    poses = orbit_poses(4)
+    frames = [sa.Frame(device(), 640, 360) for _ in poses]
+    case.renderer.render_views(frames, [p[0] for p in poses], [p[1] for p in poses], case.scene)
+    batched = [f.read() for f in frames]
+    for (cam, view), img in zip(poses, batched):
+        case.renderer.render_frame(case.frame, cam, view, case.scene)
+        assert np.array_equal(case.frame.read(), img)

@@ -1,0 +1,153 @@
+"""Pins of the oracle's camera math and CPU depth-sort fallback.
+
+renderer.rs:364-396, utils.rs:45-70.  Checks: closed-form projection, the
+dev-test scene's known answer (ties broken by index), brute force in float64
+away from the culling boundary, invariants of the sorted entries, numpy's
+stable argsort as an independent stable sort, and a planted mistake
+(division in place of reciprocal-multiply) that the key must not follow.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from splatter_amd.synth import Workload, look_at, sphere_pose
+
+NEAR, FAR = 1.0, 1000.0
+
+
+def test_perspective_closed_form():
+    out = np.zeros(16, np.float32)
+    ob.lib().orc_perspective_projection(1.0, 1.0, 1.0, 1000.0, ob.ptr(out))
+    P = out.reshape(4, 4)  # P[c] = column c
+    assert P[0, 0] == 1 and P[1, 1] == 1 and P[2, 3] == 1
+    assert P[2, 2] == np.float32(1000.0 / 999.0)
+    assert abs(P[3, 2] - (-1000.0 / 999.0)) < 1e-6
+    assert np.count_nonzero(P) == 5
+
+
+def test_fov_90_is_the_reference_value_and_60_scales_width():
+    cam, view = look_at((0, 0, -5), (0, 0, 0))
+    u = ob.uniforms(cam, view, 1920, 1080, fov=math.pi / 2)
+    assert u.view_size[1] == 1.0  # tanf(pi/4) == 1.0f: bit-identical to renderer.rs:366-368
+    assert u.view_size[0] == np.float32(1920.0 / 1080.0)
+    u60 = ob.uniforms(cam, view, 1280, 720, fov=math.radians(60))
+    assert abs(u60.view_size[1] - math.tan(math.radians(30))) < 1e-6
+    assert abs(u60.view_size[0] - (1280 / 720) * math.tan(math.radians(30))) < 1e-6
+    assert u60.ellipse_size_bias == np.float32(np.float32(0.2) * np.float32(u60.view_size[0])) / np.float32(1280)
+
+
+def test_camera_times_view_is_identity():
+    for seed in range(5):
+        cam, view = sphere_pose(seed)
+        C = cam.astype(np.float64).T  # row-major matrices
+        V = view.astype(np.float64).T
+        assert np.allclose(C @ V, np.eye(4), atol=1e-5)
+
+
+def dev_test_keys():
+    scene = ob.dev_test_scene()
+    cam, view = look_at((0, 0, -5), (0, 0, 0))
+    u = ob.uniforms(cam, view, 800, 600)
+    keys, idx = ob.cpu_depth_sort(scene["center"], ob.vp_of(u), 1.1)
+    return keys, idx
+
+
+def test_dev_test_scene_known_answer():
+    keys, idx = dev_test_keys()
+    # all three visible; e_x and e_y share z_view = 5 -> bit-identical keys,
+    # so the stable order keeps 0 before 1; e_z is farther (z_view = 6)
+    assert list(idx) == [0, 1, 2]
+    assert keys[0] == keys[1] and keys[2] > keys[0]
+    for k, zv in ((keys[0], 5.0), (keys[2], 6.0)):
+        z_ndc = FAR / (FAR - NEAR) - NEAR * FAR / ((FAR - NEAR) * zv)
+        assert abs(np.uint32(k).view(np.float32) - z_ndc) < 4e-7
+
+
+def keys_f64(pos, vp):
+    V = vp.astype(np.float64).reshape(4, 4)  # V[c] = column c
+    h = pos.astype(np.float64) @ V[:3] + V[3]
+    return h[:, :3] / h[:, 3:4], h[:, 3]
+
+
+def test_brute_force_float64_matches_away_from_boundary():
+    rng = np.random.default_rng(1)
+    pos = rng.normal(0, 2, size=(4000, 3)).astype(np.float32)
+    cam, view = sphere_pose(3)
+    u = ob.uniforms(cam, view, 640, 480)
+    vp = ob.vp_of(u)
+    keys, idx = ob.cpu_depth_sort(pos, vp, 1.1)
+    clip, w = keys_f64(pos, vp)
+    margin = 1e-4
+    vis = (np.abs(clip[:, 0]) < 1.1) & (np.abs(clip[:, 1]) < 1.1) & (np.abs(clip[:, 2] - 0.5) < 0.5)
+    clear = ((np.abs(np.abs(clip[:, 0]) - 1.1) > margin) & (np.abs(np.abs(clip[:, 1]) - 1.1) > margin)
+             & (np.abs(np.abs(clip[:, 2] - 0.5) - 0.5) > margin))
+    got = np.zeros(len(pos), bool)
+    got[idx] = True
+    assert np.array_equal(got[clear], vis[clear])
+    # order = ascending f64 depth (distinct depths)
+    z = clip[idx, 2]
+    assert np.all(np.diff(z) >= -1e-6)
+    assert np.allclose(keys.view(np.float32), clip[idx, 2], rtol=1e-5, atol=1e-6)
+
+
+def test_invariants_and_stability_on_tie_heavy_input():
+    rng = np.random.default_rng(7)
+    base = rng.normal(0, 2, size=(500, 3)).astype(np.float32)
+    pos = base[rng.integers(0, 500, size=20000)]  # every centre repeated ~40 times
+    cam, view = sphere_pose(11)
+    vp = ob.vp_of(ob.uniforms(cam, view, 1920, 1080))
+    keys, idx = ob.cpu_depth_sort(pos, vp, 1.1)
+    assert len(idx) > 1000
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    ties = np.diff(keys.astype(np.int64)) == 0
+    assert ties.sum() > 1000
+    assert np.all(np.diff(idx.astype(np.int64))[ties] > 0)  # stable: index order inside ties
+    assert len(np.unique(idx)) == len(idx)
+    # the visible set is the filter of renderer.rs:383-385 on the same keys
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(order, np.arange(len(keys)))
+
+
+def test_merge_sort_equals_numpy_stable_argsort():
+    rng = np.random.default_rng(2)
+    for n in (0, 1, 2, 3, 17, 1000, 4097):
+        k = rng.integers(0, 50, size=n).astype(np.uint32)
+        v = np.arange(n, dtype=np.uint32) * 7
+        kk, vv = k.copy(), v.copy()
+        scratch = np.empty(max(2 * n, 1), np.uint32)
+        ob.lib().orc_stable_sort_pairs(ob.ptr(kk), ob.ptr(vv), n, ob.ptr(scratch))
+        o = np.argsort(k, kind="stable")
+        assert np.array_equal(kk, k[o]) and np.array_equal(vv, v[o])
+
+
+def test_key_uses_reciprocal_multiply_not_division():
+    """Planted mistake: clip = h / h.w instead of h * (1/h.w) (renderer.rs:382).
+    The two differ in the last bit for some splats; the oracle must follow
+    the reference's spelling."""
+    w = Workload("gaussian", 200000, seed=5).all()
+    pos = w["center"].astype(np.float32)
+    cam, view = sphere_pose(2)
+    vp = ob.vp_of(ob.uniforms(cam, view, 1920, 1080))
+    keys, idx = ob.cpu_depth_sort(pos, vp, 1.1)
+    V = vp.reshape(4, 4)
+    p = pos[idx]
+    h = [((V[0, r] * p[:, 0] + V[1, r] * p[:, 1]) + V[2, r] * p[:, 2]) + V[3, r] * np.float32(1) for r in range(4)]
+    recip = (np.float32(1) / h[3]).astype(np.float32)
+    z_mul = (h[2] * recip).astype(np.float32)
+    z_div = (h[2] / h[3]).astype(np.float32)
+    assert np.array_equal(keys, z_mul.view(np.uint32))
+    assert np.count_nonzero(z_mul != z_div) > 0
+
+
+def test_vertex_stage_predicate_differs_from_fallback_only_at_the_boundary():
+    """DepthSorting::None culls with h/(h.w + 1e-7) (shaders.wgsl:54-61)."""
+    w = Workload("gaussian", 50000, seed=9).all()
+    cam, view = sphere_pose(4)
+    vp = ob.vp_of(ob.uniforms(cam, view, 1280, 720))
+    a = ob.gpu_predicate_cull(w["center"], vp, 1.1)
+    _, b = ob.cpu_depth_sort(w["center"], vp, 1.1)
+    assert np.all(np.diff(a.astype(np.int64)) > 0)
+    sym = np.setxor1d(a, np.sort(b))
+    assert len(sym) <= 3
