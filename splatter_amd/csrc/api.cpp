@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -103,9 +104,12 @@ struct spl_renderer {
   bool last_sorted = false;
   int last_final = 0;
   bool have_frame = false;
+  // stage timing: one set of 8 events per frame, read out in bulk
   bool timing = false;
-  hipEvent_t ev[8] = {};
-  float stage_ms[7] = {};
+  std::vector<std::array<hipEvent_t, 8>> ev_pool;
+  size_t ev_used = 0;
+  double stage_sum_ms[7] = {};
+  uint64_t stage_frames = 0;
 };
 
 extern "C" {
@@ -189,8 +193,11 @@ static void renderer_free(spl_renderer* r) {
   dev_free(r->pair_hist);
   dev_free(r->ranges);
   dev_free(r->conics);
-  for (auto& e : r->ev)
-    if (e) (void)hipEventDestroy(e), e = nullptr;
+  for (auto& set : r->ev_pool)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
+  r->ev_pool.clear();
+  r->ev_used = 0;
 }
 
 int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_renderer** out) {
@@ -291,16 +298,17 @@ int spl_scene_create(spl_renderer* r, uint64_t splat_count, spl_scene** out) {
   s->count = std::min<uint64_t>(splat_count, r->cap);  // scene.rs:46
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
-  chk(dev_alloc(&s->centers, s->count * 3));
-  chk(dev_alloc(&s->rotations, s->count));
-  chk(dev_alloc(&s->scale_alpha, s->count));
-  chk(dev_alloc(&s->sh, s->count * 48));
+  const uint64_t slots = std::max<uint64_t>(s->count, 1);
+  chk(dev_alloc(&s->centers, slots * 3));
+  chk(dev_alloc(&s->rotations, slots));
+  chk(dev_alloc(&s->scale_alpha, slots));
+  chk(dev_alloc(&s->sh, slots * 48));
   if (e == hipSuccess) {
     // wgpu zero-initialises new buffers; so do we
-    chk(hipMemset(s->centers, 0, std::max<uint64_t>(s->count, 1) * 12));
-    chk(hipMemset(s->rotations, 0, std::max<uint64_t>(s->count, 1) * 16));
-    chk(hipMemset(s->scale_alpha, 0, std::max<uint64_t>(s->count, 1) * 16));
-    chk(hipMemset(s->sh, 0, std::max<uint64_t>(s->count, 1) * 192));
+    chk(hipMemset(s->centers, 0, slots * 12));
+    chk(hipMemset(s->rotations, 0, slots * 16));
+    chk(hipMemset(s->scale_alpha, 0, slots * 16));
+    chk(hipMemset(s->sh, 0, slots * 192));
   }
   if (e != hipSuccess) {
     dev_free(s->centers);
@@ -607,8 +615,16 @@ static int enqueue_frame(spl_renderer* r, spl_scene* s, spl_frame* f, const floa
   const int cus = r->dev->cu_count;
   const size_t sort_words = (size_t)div_up(n, kSortThreads * kSortItems) * radix;
   const size_t cull_words = div_up(n, kCullThreads * kCullItems);
-  const bool timing = r->timing;
-  auto mark = [&](int i) { if (timing) (void)hipEventRecord(r->ev[i], q); };
+  hipEvent_t* ev = nullptr;
+  if (r->timing) {
+    if (r->ev_used == r->ev_pool.size()) {
+      std::array<hipEvent_t, 8> set{};
+      for (auto& e : set) HIP_TRY(hipEventCreate(&e));
+      r->ev_pool.push_back(set);
+    }
+    ev = r->ev_pool[r->ev_used++].data();
+  }
+  auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], q); };
 
   mark(0);
   {
@@ -763,13 +779,7 @@ int spl_render_frame(spl_renderer* r, spl_scene* s, spl_frame* f, const float ca
   if (s->dev != r->dev || f->dev != r->dev) return fail(SPL_ERROR_INVALID_ARGUMENT, "scene, frame and renderer must share a device");
   if (s->count > r->cap) return fail(SPL_ERROR_CAPACITY, "scene has more splats than max_splat_count");
   HIP_TRY(hipSetDevice(r->dev->ordinal));
-  int rc = enqueue_frame(r, s, f, cam, view);
-  if (rc) return rc;
-  if (r->timing) {
-    HIP_TRY(hipEventSynchronize(r->ev[7]));
-    for (int i = 0; i < 7; ++i) (void)hipEventElapsedTime(&r->stage_ms[i], r->ev[i], r->ev[i + 1]);
-  }
-  return SPL_OK;
+  return enqueue_frame(r, s, f, cam, view);
 }
 
 int spl_render_views(spl_renderer* r, spl_scene* s, uint32_t view_count, spl_frame* const* frames,
@@ -896,16 +906,29 @@ int spl_debug_read_conics(spl_renderer* r, double* out, uint64_t capacity, uint6
 int spl_debug_enable_timing(spl_renderer* r, int enable) {
   if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
   HIP_TRY(hipSetDevice(r->dev->ordinal));
-  if (enable)
-    for (auto& e : r->ev)
-      if (!e) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
   r->timing = enable != 0;
+  r->ev_used = 0;
+  for (double& v : r->stage_sum_ms) v = 0.0;
+  r->stage_frames = 0;
   return SPL_OK;
 }
 
 int spl_debug_stage_times(spl_renderer* r, float* ms, uint32_t capacity) {
   if (!r || !ms) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
-  for (uint32_t i = 0; i < capacity && i < 7; ++i) ms[i] = r->stage_ms[i];
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  for (size_t f = 0; f < r->ev_used; ++f) {
+    for (int i = 0; i < 7; ++i) {
+      float t = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&t, r->ev_pool[f][i], r->ev_pool[f][i + 1]));
+      r->stage_sum_ms[i] += t;
+    }
+  }
+  r->stage_frames += r->ev_used;
+  r->ev_used = 0;
+  for (uint32_t i = 0; i < capacity && i < 7; ++i)
+    ms[i] = r->stage_frames ? (float)(r->stage_sum_ms[i] / (double)r->stage_frames) : 0.0f;
   return SPL_OK;
 }
 
