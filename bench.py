@@ -152,6 +152,13 @@ def main() -> int:
         Vs.append(r.visible_count())
         Ks.append(r.pair_count())
     V, K = float(np.mean(Vs)), float(np.mean(Ks))
+    r.enable_tile_stats(True)
+    r.render_frame(frames[0], *poses[0], scene)
+    ts = r.tile_stats()
+    r.enable_tile_stats(False)
+    tiles_info = {"max_list": int(ts[:, 0].max()), "mean_list": round(float(ts[:, 0].mean()), 1),
+                  "max_consumed": int(ts[:, 1].max()), "consumed_total": int(ts[:, 1].sum()),
+                  "pairs_total": int(ts[:, 0].sum())}
     tiles = math.ceil(W / 16) * math.ceil(H / 16)
     P = math.ceil(30 / cfg.radix_bits_per_digit)
     Q = 1 if tiles <= 256 else (2 if tiles <= 65536 else 3)
@@ -195,6 +202,7 @@ def main() -> int:
         "visible_per_view": int(V),
         "pairs_per_view": int(K),
         "stages": per_stage,
+        "raster_tiles": tiles_info,
         "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_frame": launches,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},

@@ -166,6 +166,12 @@ int spl_debug_read_conics(spl_renderer* renderer, double* conics, uint64_t capac
 /* Number of (splat, screen tile) pairs of the last frame. */
 int spl_debug_pair_count(spl_renderer* renderer, uint64_t* out);
 
+/* Per screen tile of the last frame: its (splat, tile) list length and how
+ * many of those entries the rasteriser consumed before every pixel of the
+ * tile was opaque (2 x u32 per tile, row-major tiles of 16x16 pixels). */
+int spl_debug_enable_tile_stats(spl_renderer* renderer, int enable);
+int spl_debug_read_tile_stats(spl_renderer* renderer, uint32_t* out, uint64_t capacity, uint64_t* count);
+
 /* ---- timing taps for the harness ---------------------------------------- */
 /* Per-stage device time of the last frame in milliseconds, measured with HIP
  * events on the renderer's stream when enabled: cull, sort, project, bin,

@@ -222,6 +222,17 @@ class Renderer:
         check(lib().spl_debug_read_conics(self._h, out.ctypes.data_as(C.c_void_p), n, C.byref(cnt)))
         return out[:cnt.value]
 
+    def enable_tile_stats(self, enable: bool = True) -> None:
+        check(lib().spl_debug_enable_tile_stats(self._h, int(enable)))
+
+    def tile_stats(self) -> np.ndarray:
+        """(tiles, 2) array: list length and entries consumed per 16x16 tile."""
+        cnt = C.c_uint64()
+        check(lib().spl_debug_read_tile_stats(self._h, None, 0, C.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), 2), np.uint32)
+        check(lib().spl_debug_read_tile_stats(self._h, out.ctypes.data_as(C.c_void_p), cnt.value, C.byref(cnt)))
+        return out[:cnt.value]
+
     def enable_timing(self, enable: bool = True) -> None:
         check(lib().spl_debug_enable_timing(self._h, int(enable)))
 

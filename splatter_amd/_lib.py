@@ -63,6 +63,8 @@ SIGNATURES = [
     ("spl_debug_enable_conics", C.c_int, [_P, C.c_int]),
     ("spl_debug_read_conics", C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("spl_debug_pair_count", C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    ("spl_debug_enable_tile_stats", C.c_int, [_P, C.c_int]),
+    ("spl_debug_read_tile_stats", C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("spl_debug_enable_timing", C.c_int, [_P, C.c_int]),
     ("spl_debug_stage_times", C.c_int, [_P, _F16, C.c_uint32]),
 ]

@@ -100,6 +100,10 @@ struct spl_renderer {
   uint32_t ranges_cap = 0;
   double* conics = nullptr;
   bool conics_on = false;
+  uint2* tile_work = nullptr;
+  uint32_t tile_work_cap = 0;
+  bool tile_stats_on = false;
+  uint32_t last_tiles = 0;
   // last frame
   bool last_sorted = false;
   int last_final = 0;
@@ -193,6 +197,7 @@ static void renderer_free(spl_renderer* r) {
   dev_free(r->pair_hist);
   dev_free(r->ranges);
   dev_free(r->conics);
+  dev_free(r->tile_work);
   for (auto& set : r->ev_pool)
     for (auto& e : set)
       if (e) (void)hipEventDestroy(e), e = nullptr;
@@ -606,6 +611,14 @@ static int enqueue_frame(spl_renderer* r, spl_scene* s, spl_frame* f, const floa
   const uint32_t tiles = u.tiles_x * u.tiles_y;
   int rc = ensure_ranges(r, tiles);
   if (rc) return rc;
+  if (r->tile_stats_on && r->tile_work_cap < tiles) {
+    HIP_TRY(hipStreamSynchronize(q));
+    dev_free(r->tile_work);
+    r->tile_work_cap = 0;
+    HIP_TRY(dev_alloc(&r->tile_work, tiles));
+    r->tile_work_cap = tiles;
+  }
+  r->last_tiles = tiles;
   const uint32_t n = (uint32_t)s->count;
   const bool sorted = cfg.depth_sorting != SPL_DEPTH_SORTING_NONE;
   const int P = sorted ? r->depth_passes : 0;
@@ -764,6 +777,7 @@ static int enqueue_frame(spl_renderer* r, spl_scene* s, spl_frame* f, const floa
     a.blend_f32 = cfg.blend_mode == SPL_BLEND_F32 ? 1 : 0;
     a.bgra = cfg.frame_format == SPL_FRAME_FORMAT_BGRA8_UNORM ? 1 : 0;
     a.pixels = f->pixels;
+    a.tile_work = r->tile_stats_on ? r->tile_work : nullptr;
     launch_raster(a, q);
   }
   mark(7);
@@ -900,6 +914,24 @@ int spl_debug_read_conics(spl_renderer* r, double* out, uint64_t capacity, uint6
   const size_t m = (size_t)std::min<uint64_t>(capacity, order.size());
   if (out)
     for (size_t i = 0; i < m; ++i) std::memcpy(out + 6 * i, &cn[(size_t)order[i] * 6], 48);
+  return SPL_OK;
+}
+
+int spl_debug_enable_tile_stats(spl_renderer* r, int enable) {
+  if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  r->tile_stats_on = enable != 0;
+  return SPL_OK;
+}
+
+int spl_debug_read_tile_stats(spl_renderer* r, uint32_t* out, uint64_t capacity, uint64_t* count) {
+  if (!r || !count) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  if (!r->tile_stats_on || !r->tile_work || !r->have_frame)
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "tile statistics not enabled");
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  *count = r->last_tiles;
+  const size_t m = (size_t)std::min<uint64_t>(capacity, r->last_tiles);
+  if (out && m) HIP_TRY(hipMemcpy(out, r->tile_work, m * sizeof(uint2), hipMemcpyDeviceToHost));
   return SPL_OK;
 }
 

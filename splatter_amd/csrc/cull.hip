@@ -85,8 +85,8 @@ __global__ __launch_bounds__(THREADS) void k_cull_key_compact(CullLaunch a) {
       }
       if (lane < ITEMS * WAVES) s_cnt[lane] = x - v;
       const uint32_t total = __shfl(x, kWave - 1, kWave);
+      const uint32_t prefix = wave_lookback(a.status, t, total);
       if (lane == 0) {
-        const uint32_t prefix = lookback(a.status, t, 1, total);
         s_prefix = prefix;
         if (t == num_tiles - 1) a.counters[kVisible] = prefix + total;
       }

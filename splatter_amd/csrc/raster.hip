@@ -104,11 +104,13 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
       s_off[tid * ITEMS + k] = excl;
       excl += cnt[k];
     }
-    if (tid == 0) {
-      s_off[TILE] = total;
-      const uint64_t prefix = lookback64(a.status, t, (uint64_t)total);
-      s_prefix = prefix;
-      if (t == num_tiles - 1) {
+    if (tid < kWave) {
+      const uint64_t prefix = wave_lookback64(a.status, t, (uint64_t)total);
+      if (tid == 0) {
+        s_off[TILE] = total;
+        s_prefix = prefix;
+      }
+      if (tid == 0 && t == num_tiles - 1) {
         const uint64_t k_all = prefix + total;
         a.counters[kPairs] = (uint32_t)(k_all < a.pair_cap ? k_all : a.pair_cap);
         a.counters[kPairOverflow] = k_all > a.pair_cap ? 1u : 0u;
@@ -152,11 +154,21 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
   }
 }
 
+// One 256-lane block per 16x16 tile; wave w owns the 8x8 quadrant
+// (w & 1, w >> 1), one lane per pixel.  Entries arrive 256 at a time in draw
+// order: every lane gathers one record and stores the prepared entry (inverse
+// affine map, translation, coverage half-extent, colour, pixel box) in LDS,
+// while the next batch is already being fetched into registers.  Each wave
+// then keeps, in order, only the entries whose pixel box meets its quadrant
+// (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
+// stops blending, and the block stops when all four have.
 __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   constexpr int B = 256;
-  __shared__ float s_i00[B], s_i01[B], s_i10[B], s_i11[B], s_tx[B], s_ty[B], s_half[B];
-  __shared__ float s_r[B], s_g[B], s_b[B], s_a[B];
-  __shared__ uint32_t s_lo[B], s_hi[B];
+  __shared__ float4 s_m[2][B];     // i00 i01 i10 i11
+  __shared__ float4 s_t[2][B];     // tx ty half alpha
+  __shared__ float4 s_c[2][B];     // r g b -
+  __shared__ uint2 s_box[2][B];    // pixel box lo / hi (x | y << 16)
+  __shared__ uint8_t s_list[4][B]; // per-wave compacted entry slots
 
   const Uniforms& u = a.u;
   const uint32_t T = u.tiles_x * u.tiles_y;
@@ -165,9 +177,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   const uint32_t per_xcd = gridDim.x / 8;
   const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
   if (tile >= T) return;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t px = (tile % u.tiles_x) * kTileSize + (tid & 15u);
-  const uint32_t py = (tile / u.tiles_x) * kTileSize + (tid >> 4);
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t qx0 = (tile % u.tiles_x) * kTileSize + (w & 1u) * 8u;
+  const uint32_t qy0 = (tile / u.tiles_x) * kTileSize + (w >> 1) * 8u;
+  const uint32_t px = qx0 + (lane & 7u), py = qy0 + (lane >> 3);
   const uint32_t W = u.image_size[0], H = u.image_size[1];
   const bool inside = px < W && py < H;
   const uint2 range = a.ranges[tile];
@@ -176,55 +189,84 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
 
   const float qx = ((((float)px + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
   const float qy = (1.0f - (((float)py + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
-  const uint32_t row0 = (tile / u.tiles_x) * kTileSize + (tid >> 6) * 4, row1 = row0 + 3;
   const bool f32 = a.blend_f32 != 0;
   const bool unaligned = a.use_unaligned_rectangles != 0;
   const float margin = u.ellipse_margin;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = f32 ? 1.0f : 255.0f;
-  bool done = !inside;
-
   const uint32_t visible = *a.visible;
-  for (uint32_t base = range.x; base < range.y; base += B) {
-    const uint32_t cnt = (range.y - base) < (uint32_t)B ? (range.y - base) : (uint32_t)B;
-    if (tid < cnt) {
+
+  // register stage of one entry per lane
+  float4 g0 = make_float4(0, 0, 0, 0), g1 = g0, g2 = g0;
+  auto fetch = [&](uint32_t base) {
+    if (base + tid < range.y) {
       uint32_t c = a.pair_vals[base + tid];
       if (c >= visible) c = 0;  // only after an upstream error; never read past the records
-      const float4 r0 = a.records[3 * (size_t)c + 0];
-      const float4 r1 = a.records[3 * (size_t)c + 1];
-      const float4 r2 = a.records[3 * (size_t)c + 2];
-      const float A00 = r1.x, A10 = r1.y, A01 = r1.z, A11 = r1.w;
-      const float det = A00 * A11 - A10 * A01;
-      const float rdet = 1.0f / det;
-      s_i00[tid] = A11 * rdet;
-      s_i10[tid] = -A10 * rdet;
-      s_i01[tid] = -A01 * rdet;
-      s_i11[tid] = A00 * rdet;
-      s_tx[tid] = r2.x;
-      s_ty[tid] = r2.y;
-      s_half[tid] = unaligned ? margin : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
-      s_r[tid] = r0.x; s_g[tid] = r0.y; s_b[tid] = r0.z; s_a[tid] = r0.w;
-      s_lo[tid] = __float_as_uint(r2.z);
-      s_hi[tid] = __float_as_uint(r2.w);
+      g0 = a.records[3 * (size_t)c + 0];
+      g1 = a.records[3 * (size_t)c + 1];
+      g2 = a.records[3 * (size_t)c + 2];
     }
-    __syncthreads();
-    if (!done) {
-      for (uint32_t j = 0; j < cnt; ++j) {
-        const uint32_t lo = s_lo[j], hi = s_hi[j];
-        const uint32_t y0 = lo >> 16, y1 = hi >> 16;
-        if (row1 < y0 || row0 > y1) continue;  // wave-uniform
-        const uint32_t x0 = lo & 0xFFFFu, x1 = hi & 0xFFFFu;
-        if (px < x0 || px > x1 || py < y0 || py > y1 || tr == 0.0f) continue;
-        const float dx = qx - s_tx[j], dy = qy - s_ty[j];
-        const float tcx = s_i00[j] * dx + s_i01[j] * dy;
-        const float tcy = s_i10[j] * dx + s_i11[j] * dy;
-        const float h = s_half[j];
+  };
+  auto stage = [&](int buf) {
+    const float A00 = g1.x, A10 = g1.y, A01 = g1.z, A11 = g1.w;
+    const float det = A00 * A11 - A10 * A01;
+    const float rdet = 1.0f / det;
+    s_m[buf][tid] = make_float4(A11 * rdet, -A01 * rdet, -A10 * rdet, A00 * rdet);
+    const float half = unaligned ? margin : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
+    s_t[buf][tid] = make_float4(g2.x, g2.y, half, g0.w);
+    s_c[buf][tid] = make_float4(g0.x, g0.y, g0.z, 0.0f);
+    s_box[buf][tid] = make_uint2(__float_as_uint(g2.z), __float_as_uint(g2.w));
+  };
+
+  bool wave_done = __ballot(inside) == 0ull;
+  uint32_t consumed = 0;
+  int buf = 0;
+  if (range.x < range.y) {
+    fetch(range.x);
+    stage(0);
+  }
+  __syncthreads();
+  for (uint32_t base = range.x; base < range.y; base += B) {
+    const uint32_t cnt = (range.y - base) < (uint32_t)B ? (range.y - base) : (uint32_t)B;
+    const bool more = base + B < range.y;
+    if (more) fetch(base + B);
+    consumed += cnt;
+    if (!wave_done) {
+      // entries of this batch whose pixel box meets the quadrant, in order
+      uint32_t n = 0;
+#pragma unroll
+      for (int k = 0; k < B / 64; ++k) {
+        const uint32_t j = k * 64 + lane;
+        bool ov = false;
+        if (j < cnt) {
+          const uint2 bx = s_box[buf][j];
+          const uint32_t x0 = bx.x & 0xFFFFu, y0 = bx.x >> 16, x1 = bx.y & 0xFFFFu, y1 = bx.y >> 16;
+          ov = bx.x != 0xFFFFFFFFu && x0 <= qx0 + 7u && x1 >= qx0 && y0 <= qy0 + 7u && y1 >= qy0;
+        }
+        const uint64_t m = __ballot(ov);
+        if (ov) s_list[w][n + lanes_below(m)] = (uint8_t)j;
+        n += (uint32_t)__popcll(m);
+      }
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t j = s_list[w][i];
+        const uint2 bx = s_box[buf][j];
+        if (tr == 0.0f || px < (bx.x & 0xFFFFu) || px > (bx.y & 0xFFFFu) || py < (bx.x >> 16) ||
+            py > (bx.y >> 16))
+          continue;
+        const float4 m = s_m[buf][j];
+        const float4 t = s_t[buf][j];
+        const float dx = qx - t.x, dy = qy - t.y;
+        const float tcx = m.x * dx + m.y * dy;
+        const float tcy = m.z * dx + m.w * dy;
+        const float h = t.z;
         const bool in = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
                                   : (dx >= -h && dx < h && dy >= -h && dy < h);
         if (!in) continue;
         const float power = tcx * tcx + tcy * tcy;
-        const float alpha = s_a[j] * spl_exp(-0.5f * power);
+        const float alpha = t.w * spl_exp(-0.5f * power);
         if (!(alpha >= 1.0f / 255.0f)) continue;
-        const float sr = s_r[j] * alpha, sg = s_g[j] * alpha, sb = s_b[j] * alpha;
+        const float4 col = s_c[buf][j];
+        const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
         if (!f32) {
           const float da = tr / 255.0f;
           const float sa = clamp01(alpha);
@@ -240,10 +282,13 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           tr = da * (1.0f - alpha);
         }
       }
-      done = tr == 0.0f;
+      wave_done = __ballot(inside && tr != 0.0f) == 0ull;
     }
-    if (__syncthreads_and(done)) break;
+    if (more) stage(buf ^ 1);
+    buf ^= 1;
+    if (__syncthreads_and(wave_done)) break;
   }
+  if (a.tile_work && tid == 0) a.tile_work[tile] = make_uint2(range.y - range.x, consumed);
   if (inside) {
     uint32_t r, g, b, al;
     if (!f32) {

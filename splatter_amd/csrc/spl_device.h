@@ -31,33 +31,74 @@ __device__ __forceinline__ void store_status(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Decoupled look-back for one counter of tile `tile`: publishes the
-// aggregate, walks back to the nearest inclusive prefix, publishes the
-// inclusive prefix and returns the exclusive prefix.  `stride` is the
-// distance between consecutive tiles' words.  Tiles are taken in order from
-// an atomic counter by resident blocks, so every predecessor is running or
-// done and the wait ends.
-__device__ __forceinline__ uint32_t lookback(uint32_t* status, uint32_t tile, size_t stride, uint32_t count,
-                                             bool publish_aggregate = true) {
+// Look-back for one digit counter that inspects WIN predecessors per step
+// (their loads are issued together), so the wait for an inclusive prefix
+// that is t tiles away costs t/WIN memory round trips instead of t.
+// Publishes nothing: the caller publishes the aggregate beforehand and the
+// inclusive prefix (returned + count) afterwards.
+template <int WIN>
+__device__ __forceinline__ uint32_t lookback_window(const uint32_t* status, uint32_t tile, size_t stride) {
+  uint32_t prefix = 0;
+  int64_t p = (int64_t)tile - 1;
+  while (p >= 0) {
+    uint32_t s[WIN];
+#pragma unroll
+    for (int i = 0; i < WIN; ++i) s[i] = (p - i >= 0) ? load_status(status + (size_t)(p - i) * stride) : kFlagInclusive;
+    int i = 0;
+    bool stop = false;
+    for (; i < WIN; ++i) {
+      if ((s[i] & kFlagMask) == 0) break;
+      prefix += s[i] & kCountMask;
+      if (s[i] & kFlagInclusive) { stop = true; break; }
+    }
+    if (stop) break;
+    p -= i;
+    if (i < WIN) __builtin_amdgcn_s_sleep(1);
+  }
+  return prefix;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Look-back for a single counter per tile run by one whole wave: the 64
+// lanes read the 64 nearest predecessors at once and a ballot finds the
+// nearest inclusive prefix (or the first not yet published).  Returns the
+// exclusive prefix in every lane; lane 0 publishes aggregate and inclusive.
+__device__ __forceinline__ uint32_t wave_lookback(uint32_t* status, uint32_t tile, uint32_t count) {
+  const uint32_t lane = lane_id();
   if (tile == 0) {
-    store_status(status, kFlagInclusive | count);
+    if (lane == 0) store_status(status, kFlagInclusive | count);
     return 0;
   }
-  uint32_t* mine = status + (size_t)tile * stride;
-  if (publish_aggregate) store_status(mine, kFlagAggregate | count);
+  if (lane == 0) store_status(status + tile, kFlagAggregate | count);
   uint32_t prefix = 0;
-  int64_t prev = (int64_t)tile - 1;
+  int64_t base = (int64_t)tile - 1;
   while (true) {
-    uint32_t s = load_status(status + (size_t)prev * stride);
-    if ((s & kFlagMask) == 0) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
+    const int64_t i = base - (int64_t)lane;
+    const uint32_t s = i >= 0 ? load_status(status + i) : kFlagInclusive;
+    const uint64_t inv = __ballot((s & kFlagMask) == 0);
+    const uint64_t inc = __ballot((s & kFlagInclusive) != 0);
+    const uint32_t first_inc = inc ? (uint32_t)__builtin_ctzll(inc) : 64u;
+    const uint32_t first_inv = inv ? (uint32_t)__builtin_ctzll(inv) : 64u;
+    if (first_inc < first_inv) {
+      prefix += wave_sum(lane <= first_inc ? (s & kCountMask) : 0u);
+      break;
     }
-    prefix += s & kCountMask;
-    if (s & kFlagInclusive) break;
-    --prev;
+    prefix += wave_sum(lane < first_inv ? (s & kCountMask) : 0u);
+    base -= first_inv;
+    if (first_inv < 64u) __builtin_amdgcn_s_sleep(1);
   }
-  store_status(mine, kFlagInclusive | (prefix + count));
+  if (lane == 0) store_status(status + tile, kFlagInclusive | (prefix + count));
   return prefix;
 }
 
@@ -67,25 +108,33 @@ constexpr uint64_t kFlagAggregate64 = 1ull << 62;
 constexpr uint64_t kFlagInclusive64 = 1ull << 63;
 constexpr uint64_t kCountMask64 = (1ull << 62) - 1;
 
-__device__ __forceinline__ uint64_t lookback64(uint64_t* status, uint32_t tile, uint64_t count) {
+__device__ __forceinline__ uint64_t wave_lookback64(uint64_t* status, uint32_t tile, uint64_t count) {
+  const uint32_t lane = lane_id();
+  auto ld = [](const uint64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+  auto st = [](uint64_t* p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
   if (tile == 0) {
-    __hip_atomic_store(status, kFlagInclusive64 | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0) st(status, kFlagInclusive64 | count);
     return 0;
   }
-  __hip_atomic_store(status + tile, kFlagAggregate64 | count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0) st(status + tile, kFlagAggregate64 | count);
   uint64_t prefix = 0;
-  int64_t prev = (int64_t)tile - 1;
+  int64_t base = (int64_t)tile - 1;
   while (true) {
-    uint64_t s = __hip_atomic_load(status + prev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((s >> 62) == 0) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
+    const int64_t i = base - (int64_t)lane;
+    const uint64_t s = i >= 0 ? ld(status + i) : kFlagInclusive64;
+    const uint64_t inv = __ballot((s >> 62) == 0);
+    const uint64_t inc = __ballot((s & kFlagInclusive64) != 0);
+    const uint32_t first_inc = inc ? (uint32_t)__builtin_ctzll(inc) : 64u;
+    const uint32_t first_inv = inv ? (uint32_t)__builtin_ctzll(inv) : 64u;
+    if (first_inc < first_inv) {
+      prefix += wave_sum64(lane <= first_inc ? (s & kCountMask64) : 0ull);
+      break;
     }
-    prefix += s & kCountMask64;
-    if (s & kFlagInclusive64) break;
-    --prev;
+    prefix += wave_sum64(lane < first_inv ? (s & kCountMask64) : 0ull);
+    base -= first_inv;
+    if (first_inv < 64u) __builtin_amdgcn_s_sleep(1);
   }
-  __hip_atomic_store(status + tile, kFlagInclusive64 | (prefix + count), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0) st(status + tile, kFlagInclusive64 | (prefix + count));
   return prefix;
 }
 

@@ -133,6 +133,7 @@ struct RasterLaunch {
   int blend_f32;
   int bgra;
   uint32_t* pixels;
+  uint2* tile_work;         // optional: per tile (list length, entries consumed)
 };
 
 constexpr int kCullThreads = 256, kCullItems = 16;
