@@ -353,12 +353,15 @@ static inline void shade(const orc_prepared* p, int unaligned, float qx, float q
     if (!(alpha >= 1.0f / 255.0f)) return;
     float sr = p->r * alpha, sg = p->g * alpha, sb = p->b * alpha;
     if (blend_mode == ORC_BLEND_UNORM8) {
-        /* st holds the stored unorm8 values as floats 0..255 */
-        float da = st[3] / 255.0f;
+        /* st holds the stored unorm8 codes C, T as floats 0..255.  The ideal
+         * unorm blender stores round(255 * (s * T/255 + C/255)) =
+         * round(s * T + C) with s clamped to [0,1] and the sum to 255, and
+         * round(255 * (T/255) * (1 - a)) = round(T * (1 - a)) for alpha. */
+        float t = st[3];
         float sa = fminf(fmaxf(alpha, 0.0f), 1.0f);
         float s[3] = {fminf(fmaxf(sr, 0.0f), 1.0f), fminf(fmaxf(sg, 0.0f), 1.0f), fminf(fmaxf(sb, 0.0f), 1.0f)};
-        for (int c = 0; c < 3; ++c) st[c] = (float)q8(s[c] * da + st[c] / 255.0f);
-        st[3] = (float)q8(da * (1.0f - sa));
+        for (int c = 0; c < 3; ++c) st[c] = fminf(floorf(s[c] * t + st[c] + 0.5f), 255.0f);
+        st[3] = floorf(t * (1.0f - sa) + 0.5f);
     } else {
         float da = st[3];
         st[0] = sr * da + st[0];

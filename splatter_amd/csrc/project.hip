@@ -260,7 +260,9 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
       const double W = (double)u.image_size[0], H = (double)u.image_size[1];
       const double vw = (double)u.view_size[0], vh = (double)u.view_size[1];
       const double pcx = ((double)tx / vw + 1.0) * 0.5 * W, pcy = (1.0 - (double)ty / vh) * 0.5 * H;
-      const double rx = ex / vw * 0.5 * W + 2.0, ry = ey / vh * 0.5 * H + 2.0;
+      // +1 px: pixel centres sit half a pixel inside the box edges, the rest
+      // absorbs rounding of the f32 record and of the coverage test
+      const double rx = ex / vw * 0.5 * W + 1.0, ry = ey / vh * 0.5 * H + 1.0;
       const double fx0 = floor(pcx - rx), fx1 = ceil(pcx + rx);
       const double fy0 = floor(pcy - ry), fy1 = ceil(pcy + ry);
       if (fx1 >= 0.0 && fy1 >= 0.0 && fx0 <= W - 1.0 && fy0 <= H - 1.0) {

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   __shared__ float4 s_t[2][B];     // tx ty half alpha
   __shared__ float4 s_c[2][B];     // r g b -
   __shared__ uint2 s_box[2][B];    // pixel box lo / hi (x | y << 16)
-  __shared__ uint8_t s_list[4][B]; // per-wave compacted entry slots
+  __shared__ __align__(16) uint8_t s_list[4][B];  // per-wave compacted entry slots, read 4 at a time
 
   const Uniforms& u = a.u;
   const uint32_t T = u.tiles_x * u.tiles_y;
@@ -247,39 +247,52 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
         n += (uint32_t)__popcll(m);
       }
       __builtin_amdgcn_wave_barrier();
-      for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t j = s_list[w][i];
-        const uint2 bx = s_box[buf][j];
-        if (tr == 0.0f || px < (bx.x & 0xFFFFu) || px > (bx.y & 0xFFFFu) || py < (bx.x >> 16) ||
-            py > (bx.y >> 16))
-          continue;
-        const float4 m = s_m[buf][j];
-        const float4 t = s_t[buf][j];
-        const float dx = qx - t.x, dy = qy - t.y;
-        const float tcx = m.x * dx + m.y * dy;
-        const float tcy = m.z * dx + m.w * dy;
-        const float h = t.z;
-        const bool in = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
-                                  : (dx >= -h && dx < h && dy >= -h && dy < h);
-        if (!in) continue;
-        const float power = tcx * tcx + tcy * tcy;
-        const float alpha = t.w * spl_exp(-0.5f * power);
-        if (!(alpha >= 1.0f / 255.0f)) continue;
-        const float4 col = s_c[buf][j];
-        const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
-        if (!f32) {
-          const float da = tr / 255.0f;
-          const float sa = clamp01(alpha);
-          c0 = q8(clamp01(sr) * da + c0 / 255.0f);
-          c1 = q8(clamp01(sg) * da + c1 / 255.0f);
-          c2 = q8(clamp01(sb) * da + c2 / 255.0f);
-          tr = q8(da * (1.0f - sa));
-        } else {
-          const float da = tr;
-          c0 = sr * da + c0;
-          c1 = sg * da + c1;
-          c2 = sb * da + c2;
-          tr = da * (1.0f - alpha);
+      // four list slots per step: their LDS reads are issued together
+      for (uint32_t i = 0; i < n; i += 4) {
+        const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][i]);
+        uint2 bx[4];
+        float4 m[4], t[4], col[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t j = (js >> (8 * k)) & 255u;  // slots past n hold stale, in-range indices
+          bx[k] = s_box[buf][j];
+          m[k] = s_m[buf][j];
+          t[k] = s_t[buf][j];
+          col[k] = s_c[buf][j];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (i + k >= n) break;
+          const bool inbox = px >= (bx[k].x & 0xFFFFu) && px <= (bx[k].y & 0xFFFFu) &&
+                             py >= (bx[k].x >> 16) && py <= (bx[k].y >> 16);
+          const float dx = qx - t[k].x, dy = qy - t[k].y;
+          const float tcx = m[k].x * dx + m[k].y * dy;
+          const float tcy = m[k].z * dx + m[k].w * dy;
+          const float h = t[k].z;
+          const bool in = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
+                                    : (dx >= -h && dx < h && dy >= -h && dy < h);
+          if (inbox && in && tr != 0.0f) {
+            const float power = tcx * tcx + tcy * tcy;
+            const float alpha = t[k].w * spl_exp(-0.5f * power);
+            if (alpha >= 1.0f / 255.0f) {
+              const float sr = col[k].x * alpha, sg = col[k].y * alpha, sb = col[k].z * alpha;
+              if (!f32) {
+                // unorm8 codes C, T: round(s * T + C) (s clamped, sum capped at 255)
+                // and round(T * (1 - a)): the ideal 8-bit blender, no division
+                const float sa = clamp01(alpha);
+                c0 = fminf(floorf(clamp01(sr) * tr + c0 + 0.5f), 255.0f);
+                c1 = fminf(floorf(clamp01(sg) * tr + c1 + 0.5f), 255.0f);
+                c2 = fminf(floorf(clamp01(sb) * tr + c2 + 0.5f), 255.0f);
+                tr = floorf(tr * (1.0f - sa) + 0.5f);
+              } else {
+                const float da = tr;
+                c0 = sr * da + c0;
+                c1 = sg * da + c1;
+                c2 = sb * da + c2;
+                tr = da * (1.0f - alpha);
+              }
+            }
+          }
         }
       }
       wave_done = __ballot(inside && tr != 0.0f) == 0ull;

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
     for (int k = 0; k < DPT; ++k) {
       const uint32_t d = tid * DPT + k;
       if (d < (uint32_t)RADIX) {
-        const uint32_t prefix = t == 0 ? 0u : lookback_window<8>(a.status + d, t, RADIX);
+        const uint32_t prefix = t == 0 ? 0u : lookback_window<32>(a.status + d, t, RADIX);
         store_status(a.status + (size_t)t * RADIX + d, kFlagInclusive | (prefix + cnt_d[k]));
         s_local[d] = excl;
         s_base[d] = s_digit_start[d] + prefix - excl;

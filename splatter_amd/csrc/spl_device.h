@@ -38,22 +38,27 @@ __device__ __forceinline__ void store_status(uint32_t* p, uint32_t v) {
 // inclusive prefix (returned + count) afterwards.
 template <int WIN>
 __device__ __forceinline__ uint32_t lookback_window(const uint32_t* status, uint32_t tile, size_t stride) {
+  static_assert(WIN <= 32, "window masks are 32 bits");
   uint32_t prefix = 0;
   int64_t p = (int64_t)tile - 1;
-  while (p >= 0) {
+  while (true) {
     uint32_t s[WIN];
+    uint32_t inv = 0, inc = 0;
 #pragma unroll
-    for (int i = 0; i < WIN; ++i) s[i] = (p - i >= 0) ? load_status(status + (size_t)(p - i) * stride) : kFlagInclusive;
-    int i = 0;
-    bool stop = false;
-    for (; i < WIN; ++i) {
-      if ((s[i] & kFlagMask) == 0) break;
-      prefix += s[i] & kCountMask;
-      if (s[i] & kFlagInclusive) { stop = true; break; }
+    for (int i = 0; i < WIN; ++i) {
+      s[i] = (p - i >= 0) ? load_status(status + (size_t)(p - i) * stride) : kFlagInclusive;
+      inv |= ((s[i] & kFlagMask) == 0 ? 1u : 0u) << i;
+      inc |= ((s[i] & kFlagInclusive) != 0 ? 1u : 0u) << i;
     }
-    if (stop) break;
-    p -= i;
-    if (i < WIN) __builtin_amdgcn_s_sleep(1);
+    const uint32_t first_inv = inv ? (uint32_t)__builtin_ctz(inv) : (uint32_t)WIN;
+    const uint32_t first_inc = inc ? (uint32_t)__builtin_ctz(inc) : (uint32_t)WIN;
+    const bool done = first_inc < first_inv;
+    const uint32_t take = done ? first_inc + 1 : first_inv;  // words summed this round
+#pragma unroll
+    for (int i = 0; i < WIN; ++i) prefix += (uint32_t)i < take ? (s[i] & kCountMask) : 0u;
+    if (done) break;
+    p -= take;
+    if (take < (uint32_t)WIN) __builtin_amdgcn_s_sleep(1);
   }
   return prefix;
 }
