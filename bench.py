@@ -26,25 +26,30 @@ sys.path.insert(0, ROOT)
 
 METRIC = "6M-splat 1080p frames/s batched vs plain; sort Gkeys/s; p50/p99 render latency"
 HBM_PEAK_GBS = 8000.0
-STAGES = ["cull_key_compact", "depth_sort", "project", "bin", "pair_sort", "tile_ranges", "raster"]
+STAGES = ["cull_key_compact", "digit_histograms", "depth_sort", "project", "bin", "pair_sort",
+          "tile_ranges", "raster"]
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(V: float, K: float, N: int, W: int, H: int, L: int, P: int, Q: int) -> dict:
+def stage_bytes(V: float, K: float, N: int, W: int, H: int, L: int, P: int, Q: int, nv: int) -> dict:
     """Algorithmic HBM bytes per frame of each stage (what each kernel must
-    move at minimum, for its data layout)."""
+    move at minimum for its data layout).  Reads shared by the nv views of a
+    batch (centres, visibility masks, splat attributes) are charged 1/nv per
+    frame; splat attributes are charged for V splats per view (a lower bound
+    on the splats some view of the batch sees)."""
     sh = 12 * (L + 1) ** 2
     return {
-        "cull_key_compact": 12 * N + 8 * V,            # centre in; key + splat index out
-        "depth_sort": 16 * P * V,                      # key + value in and out per pass
-        "project": (4 + 12 + 32 + sh + 48) * V,        # index, centre, rot, scale+alpha, SH in; record out
-        "bin": (4 + 16) * V + 8 * K,                   # order + record box in; (tile, record) pairs out
+        "cull_key_compact": (12 * N + 4 * N) / nv + 8 * V,   # centres + visibility out; key + index out
+        "digit_histograms": 4 * V,                          # keys in
+        "depth_sort": 16 * P * V,                           # key + value in and out per pass
+        "project": 4 * N / nv + (12 + 32 + sh) * V / nv + 48 * V,  # masks, attributes in; record out
+        "bin": (4 + 16) * V + 8 * K,                        # order + record box in; (tile, record) pairs out
         "pair_sort": 16 * Q * K,
         "tile_ranges": 4 * K,
-        "raster": (4 + 48) * K + 4 * W * H,            # pair value + record gather; pixels out
+        "raster": (4 + 48) * K + 4 * W * H,                 # pair value + record gather; pixels out
     }
 
 
@@ -97,7 +102,7 @@ def main() -> int:
     W, H, V_PER = args.width, args.height, args.views
     L = 3
     dev = sa.Device(0)
-    cfg = sa.Configuration(spherical_harmonics_order=L, max_splat_count=args.splats)
+    cfg = sa.Configuration(spherical_harmonics_order=L, max_splat_count=args.splats, max_views=V_PER)
     r = sa.Renderer(dev, cfg)
     wl = Workload("clustered", args.splats, seed=2)
     t0 = time.perf_counter()
@@ -118,20 +123,19 @@ def main() -> int:
         batch(k)
     dev.synchronize()
 
-    # batched: K steps of V_PER views
-    r.enable_timing(True)
-    dev.synchronize()
+    # batched: K steps of V_PER views, one graph replay per step
     t0 = time.perf_counter()
     for k in range(args.steps):
         batch(k)
     dev.synchronize()
     t_batched = time.perf_counter() - t0
-    stage_ms = r.stage_times()
-    r.enable_timing(False)
     batched_fps = args.steps * V_PER / t_batched
+    log(f"batched: {batched_fps:.1f} frames/s")
 
     # plain: the same views, one render_frame each
     n_plain = args.steps * V_PER
+    for i in range(2):
+        r.render_frame(frames[0], *poses[i], scene)
     dev.synchronize()
     t0 = time.perf_counter()
     for i in range(n_plain):
@@ -139,6 +143,20 @@ def main() -> int:
         r.render_frame(frames[i % V_PER], cam, view, scene)
     dev.synchronize()
     plain_fps = n_plain / (time.perf_counter() - t0)
+    log(f"plain: {plain_fps:.1f} frames/s")
+
+    # per-stage device time per frame: HIP events around every stage of every
+    # batch of a second timed region of the same work (direct launches)
+    r.enable_timing(True)
+    for k in range(args.steps):
+        batch(k)
+    stage_ms = r.stage_times()
+    r.enable_timing(False)
+    r.enable_timing(True)
+    for i in range(min(n_plain, 64)):
+        r.render_frame(frames[0], *poses[i % len(poses)], scene)
+    plain_stage_ms = r.stage_times()
+    r.enable_timing(False)
 
     # latency of single frames (submit -> frame ready), and V / K per view
     lat, Vs, Ks = [], [], []
@@ -162,15 +180,16 @@ def main() -> int:
     tiles = math.ceil(W / 16) * math.ceil(H / 16)
     P = math.ceil(30 / cfg.radix_bits_per_digit)
     Q = 1 if tiles <= 256 else (2 if tiles <= 65536 else 3)
-    nbytes = stage_bytes(V, K, args.splats, W, H, L, P, Q)
-    per_stage = {s: {"ms": round(float(stage_ms[i]), 4),
+    nbytes = stage_bytes(V, K, args.splats, W, H, L, P, Q, V_PER)
+    per_stage = {s: {"ms": round(float(stage_ms[i]), 4), "plain_ms": round(float(plain_stage_ms[i]), 4),
                      "GB/s": round(nbytes[s] / (stage_ms[i] * 1e-3) / 1e9, 1) if stage_ms[i] > 0 else None}
                  for i, s in enumerate(STAGES)}
     dom = max(range(len(STAGES)), key=lambda i: stage_ms[i])
     dom_name = STAGES[dom]
     launches = {"depth_sort": P, "pair_sort": Q}.get(dom_name, 1)
     achieved = nbytes[dom_name] / (stage_ms[dom] * 1e-3) / 1e9
-    sort_gkeys = V / (stage_ms[1] * 1e-3) / 1e9 if stage_ms[1] > 0 else None
+    sort_i = STAGES.index("depth_sort")
+    sort_gkeys = V / (stage_ms[sort_i] * 1e-3) / 1e9 if stage_ms[sort_i] > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -203,7 +222,7 @@ def main() -> int:
         "pairs_per_view": int(K),
         "stages": per_stage,
         "raster_tiles": tiles_info,
-        "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_frame": launches,
+        "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_batch": launches,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,

@@ -89,6 +89,8 @@ typedef struct spl_configuration {
     float field_of_view_y;                /* radians; the reference hard-codes PI/2 (renderer.rs:366) */
     spl_blend_mode blend_mode;
     uint32_t tile_pairs_per_splat;        /* capacity of (splat, screen tile) pairs, per max_splat_count */
+    uint32_t max_views;                   /* views per batched launch (1..=32); workspace scales with it */
+    int32_t use_hip_graphs;               /* replay each batch shape's launch sequence from a hipGraph */
 } spl_configuration;
 
 const char* spl_last_error(void);
@@ -142,13 +144,18 @@ int spl_render_frame(spl_renderer* renderer, spl_scene* scene, spl_frame* frame,
 
 /* Batched form (no counterpart in the reference): renders `view_count`
  * views of one scene, view i with matrices at camera_matrices + 16*i and
- * view_matrices + 16*i into frames[i].  Same outputs as view_count calls of
- * spl_render_frame. */
+ * view_matrices + 16*i into frames[i] (frames may differ in size).  Views
+ * go through the pipeline max_views at a time, each stage one launch for
+ * all of them: the cull reads each splat centre once and the projection
+ * each splat's attributes once for every view that sees it.  Same outputs
+ * as view_count calls of spl_render_frame. */
 int spl_render_views(spl_renderer* renderer, spl_scene* scene, uint32_t view_count,
                      spl_frame* const* frames, const float* camera_matrices,
                      const float* view_matrices);
 
 /* ---- parity taps (replace the commented readbacks, renderer.rs:456-464) - */
+/* The debug reads below describe view `view` (default 0) of the last batch. */
+int spl_debug_select_view(spl_renderer* renderer, uint32_t view);
 /* Visible count of the last frame (the indirect draw's instance_count,
  * shaders.wgsl:458-461). */
 int spl_debug_visible_count(spl_renderer* renderer, uint64_t* out);
@@ -173,9 +180,12 @@ int spl_debug_enable_tile_stats(spl_renderer* renderer, int enable);
 int spl_debug_read_tile_stats(spl_renderer* renderer, uint32_t* out, uint64_t capacity, uint64_t* count);
 
 /* ---- timing taps for the harness ---------------------------------------- */
-/* Per-stage device time of the last frame in milliseconds, measured with HIP
- * events on the renderer's stream when enabled: cull, sort, project, bin,
- * pair sort, ranges, raster (7 values). */
+/* Per-stage device time per frame in milliseconds, averaged over the batches
+ * rendered since timing was enabled (HIP events on the renderer's stream
+ * around every stage of every batch, divided by the batch's view count):
+ * cull, digit histograms, depth sort, project, bin, pair sort, tile ranges,
+ * raster (8 values).  While timing is enabled batches run as direct
+ * launches rather than graph replays. */
 int spl_debug_enable_timing(spl_renderer* renderer, int enable);
 int spl_debug_stage_times(spl_renderer* renderer, float* ms, uint32_t capacity);
 

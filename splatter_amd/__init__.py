@@ -67,6 +67,8 @@ class Configuration:
     field_of_view_y: float = math.pi * 0.5
     blend_mode: BlendMode = BlendMode.UNORM8
     tile_pairs_per_splat: int = 4
+    max_views: int = 1
+    use_hip_graphs: bool = True
 
     def to_c(self) -> SplConfiguration:
         c = SplConfiguration()
@@ -83,6 +85,8 @@ class Configuration:
         c.field_of_view_y = float(np.float32(self.field_of_view_y))
         c.blend_mode = int(self.blend_mode)
         c.tile_pairs_per_splat = int(self.tile_pairs_per_splat)
+        c.max_views = int(self.max_views)
+        c.use_hip_graphs = int(bool(self.use_hip_graphs))
         return c
 
 
@@ -186,6 +190,10 @@ class Renderer:
                                      views.ctypes.data_as(C.POINTER(C.c_float))))
 
     # ---- parity taps ---------------------------------------------------
+    def select_view(self, view: int) -> None:
+        """Debug reads describe this view of the last batch."""
+        check(lib().spl_debug_select_view(self._h, int(view)))
+
     def visible_count(self) -> int:
         v = C.c_uint64()
         check(lib().spl_debug_visible_count(self._h, C.byref(v)))
@@ -236,9 +244,13 @@ class Renderer:
     def enable_timing(self, enable: bool = True) -> None:
         check(lib().spl_debug_enable_timing(self._h, int(enable)))
 
+    STAGES = ("cull_key_compact", "digit_histograms", "depth_sort", "project", "bin", "pair_sort",
+              "tile_ranges", "raster")
+
     def stage_times(self) -> np.ndarray:
-        out = (C.c_float * 7)()
-        check(lib().spl_debug_stage_times(self._h, out, 7))
+        """Per-frame device milliseconds of each stage (Renderer.STAGES)."""
+        out = (C.c_float * 8)()
+        check(lib().spl_debug_stage_times(self._h, out, 8))
         return np.array(out[:], dtype=np.float64)
 
     def close(self) -> None:

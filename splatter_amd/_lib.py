@@ -29,6 +29,8 @@ class SplConfiguration(C.Structure):
         ("field_of_view_y", C.c_float),
         ("blend_mode", C.c_int),
         ("tile_pairs_per_splat", C.c_uint32),
+        ("max_views", C.c_uint32),
+        ("use_hip_graphs", C.c_int32),
     ]
 
 
@@ -57,6 +59,7 @@ SIGNATURES = [
     ("spl_frame_read", C.c_int, [_P, _P, C.c_uint64]),
     ("spl_render_frame", C.c_int, [_P, _P, _P, _F16, _F16]),
     ("spl_render_views", C.c_int, [_P, _P, C.c_uint32, _PP, _F16, _F16]),
+    ("spl_debug_select_view", C.c_int, [_P, C.c_uint32]),
     ("spl_debug_visible_count", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("spl_debug_read_entries", C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("spl_debug_read_records", C.c_int, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -1,6 +1,8 @@
-// cull.hip -- frustum cull + depth key + order-preserving stream compaction.
+// cull.hip -- frustum cull + depth key + order-preserving stream compaction
+// for every view of a batch in one pass over the splat centres, and the
+// digit histograms that the radix passes start from.
 //
-// One pass over the splat centres replaces the CPU fallback's filter_map
+// Per view the cull replaces the CPU fallback's filter_map
 // (renderer.rs:372-392) and the key half of radixSortA (shaders.wgsl:303-336):
 //  - key: h = VP*(x,y,z,1) summed in mat4_transform's order (utils.rs:68-70),
 //    clip = h * (1/h.w) (renderer.rs:382), visible iff |x|<tol, |y|<tol,
@@ -9,109 +11,171 @@
 //  - compaction keeps splat-index order: per 64-splat wave a ballot gives the
 //    visible mask, mbcnt its intra-wave offsets, a scan over the tile's waves
 //    the intra-tile offsets and a decoupled look-back across tiles the tile
-//    offset, so entry c of the output is the c-th visible splat by index.
-//  - the digit histograms of every radix pass over the visible keys are
-//    gathered on the way (the onesweep upsweep), LDS first, then global.
+//    offset, so entry c of a view's list is its c-th visible splat by index.
+//  - each centre is read once for all views; the splat's visibility over the
+//    views (one bit per view) and each tile's per-view prefix are kept for
+//    the projection, which reads the splat's other attributes once for all
+//    the views that see it.
 #include "spl_device.h"
 #include "spl_internal.h"
 
 namespace spl {
 
 template <int THREADS, int ITEMS>
-__global__ __launch_bounds__(THREADS) void k_cull_key_compact(CullLaunch a) {
+__global__ __launch_bounds__(THREADS) void k_cull_views(CullLaunch a) {
   constexpr int WAVES = THREADS / kWave;
   constexpr int TILE = THREADS * ITEMS;
-  static_assert(ITEMS * WAVES <= kWave, "one wave scans the tile's wave counts");
-  __shared__ uint32_t s_hist[3 * 1024];
-  __shared__ uint32_t s_cnt[ITEMS * WAVES];
+  static_assert(ITEMS * WAVES == kWave, "one wave scans a view's (item, wave) counts");
+  __shared__ float s_vp[kMaxViews][16];
+  __shared__ float s_tol[kMaxViews];
+  __shared__ uint32_t s_cnt[kMaxViews][ITEMS * WAVES];
+  __shared__ uint32_t s_prefix[kMaxViews];
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_prefix;
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & (kWave - 1), w = tid / kWave;
-  const int radix = 1 << a.bits;
-  const int hist_words = a.passes * radix;
-  const uint32_t digit_mask = (uint32_t)radix - 1u;
-  for (int i = tid; i < hist_words; i += THREADS) s_hist[i] = 0u;
+  const uint32_t nv = a.views;
+  for (uint32_t i = tid; i < nv * 16; i += THREADS) s_vp[i / 16][i % 16] = a.uniforms[i / 16].view_projection_matrix[i % 16];
+  if (tid < nv) s_tol[tid] = a.uniforms[tid].frustum_culling_tolerance;
+  __syncthreads();
 
   const uint32_t n = a.n;
   const uint32_t num_tiles = (n + TILE - 1) / TILE;
-  if (num_tiles == 0 && blockIdx.x == 0 && tid == 0) a.counters[kVisible] = 0u;
+  if (num_tiles == 0 && blockIdx.x == 0 && tid < nv) a.counters[(size_t)tid * kCounterWords + kVisible] = 0u;
 
-  const float* vp = a.vp;
+  // visibility of one centre in view v, and its depth key
+  auto classify = [&](uint32_t v, float x, float y, float z, uint32_t& key) -> bool {
+    const float* vp = s_vp[v];
+    float h[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[r] = ((vp[r] * x + vp[4 + r] * y) + vp[8 + r] * z) + vp[12 + r] * 1.0f;
+    float c0, c1, c2;
+    if (a.gpu_predicate) {
+      const float wv = h[3] + 0.0000001f;
+      c0 = h[0] / wv; c1 = h[1] / wv; c2 = h[2] / wv;
+    } else {
+      const float rcp = 1.0f / h[3];
+      c0 = h[0] * rcp; c1 = h[1] * rcp; c2 = h[2] * rcp;
+    }
+    key = __float_as_uint(c2);
+    const float tol = s_tol[v];
+    return fabsf(c0) < tol && fabsf(c1) < tol && fabsf(c2 - 0.5f) < 0.5f;
+  };
+
   while (true) {
     if (tid == 0) s_tile = atomicAdd(&a.counters[kCtrCull], 1u);
     __syncthreads();
     const uint32_t t = s_tile;
     if (t >= num_tiles) break;
 
-    uint64_t vis[ITEMS];
-    uint32_t key[ITEMS];
+    float x[ITEMS], y[ITEMS], z[ITEMS];
+    uint32_t vm[ITEMS];
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t idx = t * TILE + j * THREADS + tid;
-      bool visible = false;
-      key[j] = 0u;
+      vm[j] = 0u;
+      x[j] = y[j] = z[j] = 0.0f;
       if (idx < n) {
-        const float x = a.centers[3 * (size_t)idx + 0];
-        const float y = a.centers[3 * (size_t)idx + 1];
-        const float z = a.centers[3 * (size_t)idx + 2];
-        float h[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) h[r] = ((vp[r] * x + vp[4 + r] * y) + vp[8 + r] * z) + vp[12 + r] * 1.0f;
-        float c0, c1, c2;
-        if (a.gpu_predicate) {
-          const float wv = h[3] + 0.0000001f;
-          c0 = h[0] / wv; c1 = h[1] / wv; c2 = h[2] / wv;
-        } else {
-          const float rcp = 1.0f / h[3];
-          c0 = h[0] * rcp; c1 = h[1] * rcp; c2 = h[2] * rcp;
-        }
-        visible = fabsf(c0) < a.tol && fabsf(c1) < a.tol && fabsf(c2 - 0.5f) < 0.5f;
-        key[j] = __float_as_uint(c2);
-      }
-      vis[j] = __ballot(visible);
-      if (lane == 0) s_cnt[j * WAVES + w] = (uint32_t)__popcll(vis[j]);
-    }
-    __syncthreads();
-    if (w == 0) {
-      // exclusive scan of the (item, wave) counts in splat-index order
-      const uint32_t v = lane < ITEMS * WAVES ? s_cnt[lane] : 0u;
-      uint32_t x = v;
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, kWave);
-        if (lane >= (uint32_t)o) x += y;
-      }
-      if (lane < ITEMS * WAVES) s_cnt[lane] = x - v;
-      const uint32_t total = __shfl(x, kWave - 1, kWave);
-      const uint32_t prefix = wave_lookback(a.status, t, total);
-      if (lane == 0) {
-        s_prefix = prefix;
-        if (t == num_tiles - 1) a.counters[kVisible] = prefix + total;
+        x[j] = a.centers[3 * (size_t)idx + 0];
+        y[j] = a.centers[3 * (size_t)idx + 1];
+        z[j] = a.centers[3 * (size_t)idx + 2];
       }
     }
-    __syncthreads();
-    const uint32_t prefix = s_prefix;
+    // A: visibility per (view, item), and per view the (item, wave) counts
+    for (uint32_t v = 0; v < nv; ++v) {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = t * TILE + j * THREADS + tid;
+        uint32_t key;
+        const bool vis = idx < n && classify(v, x[j], y[j], z[j], key);
+        vm[j] |= (vis ? 1u : 0u) << v;
+        const uint64_t b = __ballot(vis);
+        if (lane == 0) s_cnt[v][j * WAVES + w] = (uint32_t)__popcll(b);
+      }
+    }
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-      if ((vis[j] >> lane) & 1ull) {
-        const uint32_t pos = prefix + s_cnt[j * WAVES + w] + lanes_below(vis[j]);
-        if (a.keys_out) a.keys_out[pos] = key[j];
-        a.idx_out[pos] = t * TILE + j * THREADS + tid;
-        for (int p = 0; p < a.passes; ++p)
-          atomicAdd(&s_hist[p * radix + ((key[j] >> (p * a.bits)) & digit_mask)], 1u);
+      const uint32_t idx = t * TILE + j * THREADS + tid;
+      if (idx < n) a.vis[idx] = vm[j];
+    }
+    __syncthreads();
+    // B: per view, exclusive scan of the counts in splat-index order and the
+    // tile's look-back; wave w serves views w, w + WAVES, ...
+    for (uint32_t v = w; v < nv; v += WAVES) {
+      const uint32_t c = s_cnt[v][lane];
+      uint32_t incl = c;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y2 = __shfl_up(incl, o, kWave);
+        if (lane >= (uint32_t)o) incl += y2;
+      }
+      s_cnt[v][lane] = incl - c;
+      const uint32_t total = __shfl(incl, kWave - 1, kWave);
+      const uint32_t prefix = wave_lookback(a.status + v * a.status_stride, t, total);
+      if (lane == 0) {
+        s_prefix[v] = prefix;
+        a.tile_prefix[v * a.tp_stride + t] = prefix;
+        if (t == num_tiles - 1) a.counters[(size_t)v * kCounterWords + kVisible] = prefix + total;
+      }
+    }
+    __syncthreads();
+    // C: scatter each view's entries
+    for (uint32_t v = 0; v < nv; ++v) {
+      uint32_t* keys = a.keys + v * a.keys_stride;
+      uint32_t* idxs = a.idx + v * a.idx_stride;
+      const uint32_t pv = s_prefix[v];
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const bool vis = (vm[j] >> v) & 1u;
+        const uint64_t b = __ballot(vis);
+        if (vis) {
+          const uint32_t pos = pv + s_cnt[v][j * WAVES + w] + lanes_below(b);
+          idxs[pos] = t * TILE + j * THREADS + tid;
+          if (a.write_keys) {
+            uint32_t key;
+            (void)classify(v, x[j], y[j], z[j], key);
+            keys[pos] = key;
+          }
+        }
       }
     }
     __syncthreads();
   }
-  for (int i = tid; i < hist_words; i += THREADS)
-    if (s_hist[i]) atomicAdd(&a.hist[i], s_hist[i]);
   finish_persistent(&a.counters[kCtrCull], &a.counters[kDone + kCtrCull]);
 }
 
+// Onesweep upsweep: the digit histograms of every radix pass over a view's
+// keys, LDS first, then one global atomic per bin and block.
+template <int BITS>
+__global__ __launch_bounds__(256) void k_digit_histograms(HistLaunch a) {
+  constexpr int RADIX = 1 << BITS;
+  __shared__ uint32_t s_hist[kMaxDepthPasses * RADIX];
+  const uint32_t v = blockIdx.y;
+  if (a.clear) clear_words(a.clear + v * a.clear_stride, a.clear_words);
+  const uint32_t* keys = a.keys + v * a.keys_stride;
+  uint32_t* hist = a.hist + v * a.hist_stride;
+  const uint32_t n = a.counters[(size_t)v * kCounterWords + kVisible];
+  const int words = a.passes * RADIX;
+  for (int i = threadIdx.x; i < words; i += blockDim.x) s_hist[i] = 0u;
+  __syncthreads();
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t k = keys[i];
+    for (int p = 0; p < a.passes; ++p) atomicAdd(&s_hist[p * RADIX + ((k >> (p * BITS)) & (RADIX - 1))], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < words; i += blockDim.x)
+    if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
+}
+
 void launch_cull(const CullLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_cull_key_compact<kCullThreads, kCullItems>), dim3(a.grid), dim3(kCullThreads), 0, s, a);
+  hipLaunchKernelGGL((k_cull_views<kCullThreads, kCullItems>), dim3(a.grid), dim3(kCullThreads), 0, s, a);
+}
+
+void launch_histograms(const HistLaunch& a, hipStream_t s) {
+  if (a.bits == 10)
+    hipLaunchKernelGGL((k_digit_histograms<10>), dim3(a.grid_x, a.views), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_digit_histograms<8>), dim3(a.grid_x, a.views), dim3(256), 0, s, a);
 }
 
 }  // namespace spl

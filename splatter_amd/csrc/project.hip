@@ -58,19 +58,26 @@ __device__ __forceinline__ void quat_to_mat(const double p[4], double m[3][3]) {
   m[2][0] = yw + zx; m[2][1] = zw - yx; m[2][2] = (1.0 - yy) - zz;
 }
 
-// shaders.wgsl:125-169 projectedContourOfEllipsoid
-__device__ void contour(const double scale[3], const double rot[4], const double pos[3], const Uniforms& u,
-                        double M[3][3]) {
-  double C[3][3], T[3][3], A[3][3], Ct[3][3], Tt[3][3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) C[c][r] = (double)u.camera_matrix[4 * c + r];
+// shaders.wgsl:127-130: the view-independent half of
+// projectedContourOfEllipsoid, transform = quatToMat(rotation) / scale.
+__device__ __forceinline__ void contour_transform(const double scale[3], const double rot[4], double T[3][3]) {
   quat_to_mat(rot, T);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] / scale[c];
+}
+
+// shaders.wgsl:125-169 projectedContourOfEllipsoid, given the transform.
+__device__ void contour(const double T0[3][3], const double pos[3], const Uniforms& u, double M[3][3]) {
+  double C[3][3], T[3][3], A[3][3], Ct[3][3], Tt[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      C[c][r] = (double)u.camera_matrix[4 * c + r];
+      T[c][r] = T0[c][r];
+    }
   double ro[3], lo[3], ls[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) ro[i] = (double)u.camera_matrix[12 + i] - pos[i];
@@ -90,19 +97,23 @@ __device__ void contour(const double scale[3], const double rot[4], const double
   mat3_mul(M, Tt, M);
 }
 
-// shaders.wgsl:99-118 projectedCovarianceOfEllipsoid
-__device__ void covariance(const double scale[3], const double rot[4], const double pos[3], const Uniforms& u,
-                           double cov[3][3]) {
-  double C[3][3], T[3][3], J[3][3], Tt[3][3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) C[c][r] = (double)u.camera_matrix[4 * c + r];
+// shaders.wgsl:101-104: the view-independent half of
+// projectedCovarianceOfEllipsoid, transform = quatToMat(rotation) * scale.
+__device__ __forceinline__ void covariance_transform(const double scale[3], const double rot[4], double T[3][3]) {
   quat_to_mat(rot, T);
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] * scale[c];
+}
+
+// shaders.wgsl:99-118 projectedCovarianceOfEllipsoid, given the transform.
+__device__ void covariance(const double T[3][3], const double pos[3], const Uniforms& u, double cov[3][3]) {
+  double C[3][3], J[3][3], Tt[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) C[c][r] = (double)u.camera_matrix[4 * c + r];
   const float* V = u.view_matrix;
   double vp[4];
 #pragma unroll
@@ -168,119 +179,184 @@ __device__ __forceinline__ void sh_color(const float d[3], const float* sh, floa
 
 __device__ __forceinline__ bool finite(float x) { return (__float_as_uint(x) & 0x7F800000u) != 0x7F800000u; }
 
+// Conservative pixel box of what the rectangle can cover, in f64 so that a
+// far-off centre with a huge extent cannot cancel away.  Empty when the
+// rectangle is not finite or misses the image.
+__device__ void pixel_box(float A00, float A10, float A01, float A11, float tx, float ty, const Uniforms& u,
+                          bool unaligned, uint32_t& lo, uint32_t& hi) {
+  lo = 0xFFFFFFFFu;  // empty: x0 > x1
+  hi = 0u;
+  if (!(finite(A00) && finite(A10) && finite(A01) && finite(A11) && finite(tx) && finite(ty))) return;
+  const double m = (double)u.ellipse_margin;
+  double ex, ey;
+  if (unaligned) {
+    ex = (fabs((double)A00) + fabs((double)A01)) * m;
+    ey = (fabs((double)A10) + fabs((double)A11)) * m;
+  } else {
+    ex = sqrt(fmax((double)A00 * A00 + (double)A10 * A10, (double)A01 * A01 + (double)A11 * A11)) * m;
+    ey = ex;
+  }
+  const double W = (double)u.image_size[0], H = (double)u.image_size[1];
+  const double vw = (double)u.view_size[0], vh = (double)u.view_size[1];
+  const double pcx = ((double)tx / vw + 1.0) * 0.5 * W, pcy = (1.0 - (double)ty / vh) * 0.5 * H;
+  // +1 px: pixel centres sit half a pixel inside the box edges, the rest
+  // absorbs rounding of the f32 record and of the coverage test
+  const double rx = ex / vw * 0.5 * W + 1.0, ry = ey / vh * 0.5 * H + 1.0;
+  const double fx0 = floor(pcx - rx), fx1 = ceil(pcx + rx);
+  const double fy0 = floor(pcy - ry), fy1 = ceil(pcy + ry);
+  if (fx1 >= 0.0 && fy1 >= 0.0 && fx0 <= W - 1.0 && fy0 <= H - 1.0) {
+    const uint32_t x0 = (uint32_t)fmax(fx0, 0.0), y0 = (uint32_t)fmax(fy0, 0.0);
+    const uint32_t x1 = (uint32_t)fmin(fx1, W - 1.0), y1 = (uint32_t)fmin(fy1, H - 1.0);
+    lo = x0 | (y0 << 16);
+    hi = x1 | (y1 << 16);
+  }
+}
+
 }  // namespace
 
+// One record per (visible splat, view): the splat's attributes are read
+// once and projected into every view of the batch whose cull kept it.  Entry
+// c of view v's list (its c-th visible splat by index) is found from the
+// visibility masks and the cull's per-tile prefixes, so records land where
+// the cull put the keys, with no per-view index to read.
 template <int L>
 __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
-  clear_words(a.clear_a, a.clear_a_words);
-  clear_words(a.clear_b, a.clear_b_words);
-  const uint32_t n = *a.count;
-  const Uniforms& u = a.u;
+  constexpr int THREADS = kCullThreads, ITEMS = kCullItems, WAVES = THREADS / kWave, TILE = kCullTile;
+  __shared__ uint32_t s_off[kMaxViews][ITEMS * WAVES];
+  __shared__ uint32_t s_tp[kMaxViews];
+  const uint32_t nv = a.views;
+  for (uint32_t v = 0; v < nv; ++v) {
+    if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
+    if (a.clear_b) clear_words(a.clear_b + v * a.clear_b_stride, a.clear_b_words);
+  }
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+  const uint32_t n = a.scene.count;
+  const uint32_t num_tiles = (n + TILE - 1) / TILE;
   constexpr int SH_FLOATS = 3 * (L + 1) * (L + 1);
   constexpr int SH_F4 = (SH_FLOATS + 3) / 4;
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
-    const uint32_t i = a.compact_idx[c];
-    const float cx = a.scene.centers[3 * (size_t)i + 0];
-    const float cy = a.scene.centers[3 * (size_t)i + 1];
-    const float cz = a.scene.centers[3 * (size_t)i + 2];
-    const float4 rq = a.scene.rotations[i];
-    const float4 sa = a.scene.scale_alpha[i];
-    float sh[SH_F4 * 4];
-    const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
+
+  for (uint32_t t = blockIdx.x; t < num_tiles; t += gridDim.x) {
+    // per view: exclusive offsets of the (item, wave) groups inside the tile
+    for (uint32_t v = 0; v < nv; ++v) {
 #pragma unroll
-    for (int k = 0; k < SH_F4; ++k) {
-      const float4 v = shp[k];
-      sh[4 * k + 0] = v.x; sh[4 * k + 1] = v.y; sh[4 * k + 2] = v.z; sh[4 * k + 3] = v.w;
-    }
-
-    // shaders.wgsl:531-532: colour from the normalised ray direction
-    float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
-    const float len = sqrtf((rd[0] * rd[0] + rd[1] * rd[1]) + rd[2] * rd[2]);
-    rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
-    float rgb[3];
-    sh_color<L>(rd, sh, rgb);
-
-    // shaders.wgsl:533-547
-    const double scale[3] = {(double)sa.x * (double)u.splat_scale, (double)sa.y * (double)u.splat_scale,
-                             (double)sa.z * (double)u.splat_scale};
-    const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
-    const double pos[3] = {(double)cx, (double)cy, (double)cz};
-    double M[3][3];
-    contour(scale, rot, pos, u, M);
-    // extractTranslationOfEllipse (shaders.wgsl:179-191)
-    const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
-    const double inv_disc = 1.0 / disc;
-    const double t0 = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * inv_disc;
-    const double t1 = (M[0][1] * M[0][2] - M[0][0] * M[1][2]) * inv_disc;
-    // extractRotationOfEllipse (shaders.wgsl:193-215)
-    const double dxy = M[0][0] - M[1][1];
-    const double ra = dxy * dxy;
-    const double rb = ra + (4.0 * M[0][1]) * M[0][1];
-    const double rc = 0.5 * sqrt(ra / rb);
-    double j = sqrt(0.5 - rc);
-    double k = (-sqrt(0.5 + rc) * sgn(M[0][1])) * sgn(dxy);
-    if (M[0][1] < 0.0 || dxy < 0.0) { k = -k; j = -j; }
-    if (dxy < 0.0) { const double tt = j; j = -k; k = tt; }
-    double s0, s1;
-    if (a.use_covariance_for_scale) {
-      // extractScaleOfCovariance (shaders.wgsl:240-246)
-      double cov[3][3];
-      covariance(scale, rot, pos, u, cov);
-      const double cd = cov[0][0] - cov[1][1];
-      const double b = sqrt(cd * cd + (4.0 * cov[0][1]) * cov[0][1]);
-      s0 = sqrt(((cov[0][0] + cov[1][1]) + b) * 0.5);
-      s1 = sqrt(((cov[0][0] + cov[1][1]) - b) * 0.5);
-    } else {
-      // extractScaleOfEllipse (shaders.wgsl:217-237)
-      const double d = ((2.0 * M[0][1]) * j) * k;
-      const double e = M[2][2] - (((M[0][0] * t0) * t0 + (M[1][1] * t1) * t1) + ((2.0 * M[0][1]) * t0) * t1);
-      s0 = sqrt(fabs(e / (((M[0][0] * k) * k + (M[1][1] * j) * j) - d)));
-      s1 = sqrt(fabs(e / (((M[0][0] * j) * j + (M[1][1] * k) * k) + d)));
-    }
-    const double bias = (double)u.ellipse_size_bias;
-    const double sa0 = bias + s0, sb0 = bias + s1;
-    const float A00 = (float)(k * sa0), A10 = (float)(-j * sa0);
-    const float A01 = (float)(j * sb0), A11 = (float)(k * sb0);
-    const float tx = (float)t0, ty = (float)t1;
-
-    // conservative pixel box of the covered region
-    uint32_t box_lo = 0xFFFFFFFFu, box_hi = 0u;  // empty: x0 > x1
-    const bool ok = finite(A00) && finite(A10) && finite(A01) && finite(A11) && finite(tx) && finite(ty);
-    if (ok) {
-      // in f64 so that a far-off centre with a huge extent cannot cancel away
-      const double m = (double)u.ellipse_margin;
-      double ex, ey;
-      if (a.use_unaligned_rectangles) {
-        ex = (fabs((double)A00) + fabs((double)A01)) * m;
-        ey = (fabs((double)A10) + fabs((double)A11)) * m;
-      } else {
-        ex = sqrt(fmax((double)A00 * A00 + (double)A10 * A10, (double)A01 * A01 + (double)A11 * A11)) * m;
-        ey = ex;
-      }
-      const double W = (double)u.image_size[0], H = (double)u.image_size[1];
-      const double vw = (double)u.view_size[0], vh = (double)u.view_size[1];
-      const double pcx = ((double)tx / vw + 1.0) * 0.5 * W, pcy = (1.0 - (double)ty / vh) * 0.5 * H;
-      // +1 px: pixel centres sit half a pixel inside the box edges, the rest
-      // absorbs rounding of the f32 record and of the coverage test
-      const double rx = ex / vw * 0.5 * W + 1.0, ry = ey / vh * 0.5 * H + 1.0;
-      const double fx0 = floor(pcx - rx), fx1 = ceil(pcx + rx);
-      const double fy0 = floor(pcy - ry), fy1 = ceil(pcy + ry);
-      if (fx1 >= 0.0 && fy1 >= 0.0 && fx0 <= W - 1.0 && fy0 <= H - 1.0) {
-        const uint32_t x0 = (uint32_t)fmax(fx0, 0.0), y0 = (uint32_t)fmax(fy0, 0.0);
-        const uint32_t x1 = (uint32_t)fmin(fx1, W - 1.0), y1 = (uint32_t)fmin(fy1, H - 1.0);
-        box_lo = x0 | (y0 << 16);
-        box_hi = x1 | (y1 << 16);
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = t * TILE + j * THREADS + tid;
+        const uint32_t m = idx < n ? a.vis[idx] : 0u;
+        const uint64_t b = __ballot((m >> v) & 1u);
+        if (lane == 0) s_off[v][j * WAVES + w] = (uint32_t)__popcll(b);
       }
     }
-    float4* rec = a.records + 3 * (size_t)c;
-    rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
-    rec[1] = make_float4(A00, A10, A01, A11);
-    rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
-    if (a.conics) {
-      double* cn = a.conics + 6 * (size_t)c;
-      cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];
-      cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
+    __syncthreads();
+    for (uint32_t v = w; v < nv; v += WAVES) {
+      const uint32_t c = s_off[v][lane];
+      uint32_t incl = c;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= (uint32_t)o) incl += y;
+      }
+      s_off[v][lane] = incl - c;
+      if (lane == 0) s_tp[v] = a.tile_prefix[v * a.tp_stride + t];
     }
+    __syncthreads();
+
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t i = t * TILE + j * THREADS + tid;
+      const uint32_t m = i < n ? a.vis[i] : 0u;
+      if (__ballot(m != 0u) == 0ull) continue;
+      float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+      float4 rq = make_float4(0, 0, 0, 0), sa = rq;
+      float sh[SH_F4 * 4];
+#pragma unroll
+      for (int k = 0; k < SH_F4 * 4; ++k) sh[k] = 0.0f;
+      if (m) {
+        cx = a.scene.centers[3 * (size_t)i + 0];
+        cy = a.scene.centers[3 * (size_t)i + 1];
+        cz = a.scene.centers[3 * (size_t)i + 2];
+        rq = a.scene.rotations[i];
+        sa = a.scene.scale_alpha[i];
+        const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
+#pragma unroll
+        for (int k = 0; k < SH_F4; ++k) {
+          const float4 q = shp[k];
+          sh[4 * k + 0] = q.x; sh[4 * k + 1] = q.y; sh[4 * k + 2] = q.z; sh[4 * k + 3] = q.w;
+        }
+      }
+      // view-independent parts (shaders.wgsl:63-79,101-104,127-130)
+      const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
+      const double pos[3] = {(double)cx, (double)cy, (double)cz};
+      const double splat_scale = (double)a.uniforms[0].splat_scale;
+      const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
+      double Tc[3][3], Tv[3][3];
+      contour_transform(scale, rot, Tc);
+      if (a.use_covariance_for_scale) covariance_transform(scale, rot, Tv);
+
+      for (uint32_t v = 0; v < nv; ++v) {
+        const bool mine = (m >> v) & 1u;
+        const uint64_t bal = __ballot(mine);
+        if (!mine) continue;
+        const uint32_t c = s_tp[v] + s_off[v][j * WAVES + w] + lanes_below(bal);
+        const Uniforms& u = a.uniforms[v];
+
+        // shaders.wgsl:531-532: colour from the normalised ray direction
+        float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
+        const float len = sqrtf((rd[0] * rd[0] + rd[1] * rd[1]) + rd[2] * rd[2]);
+        rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
+        float rgb[3];
+        sh_color<L>(rd, sh, rgb);
+
+        // shaders.wgsl:533-547
+        double M[3][3];
+        contour(Tc, pos, u, M);
+        // extractTranslationOfEllipse (shaders.wgsl:179-191)
+        const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
+        const double inv_disc = 1.0 / disc;
+        const double t0 = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * inv_disc;
+        const double t1 = (M[0][1] * M[0][2] - M[0][0] * M[1][2]) * inv_disc;
+        // extractRotationOfEllipse (shaders.wgsl:193-215)
+        const double dxy = M[0][0] - M[1][1];
+        const double ra = dxy * dxy;
+        const double rb = ra + (4.0 * M[0][1]) * M[0][1];
+        const double rc = 0.5 * sqrt(ra / rb);
+        double jj = sqrt(0.5 - rc);
+        double kk = (-sqrt(0.5 + rc) * sgn(M[0][1])) * sgn(dxy);
+        if (M[0][1] < 0.0 || dxy < 0.0) { kk = -kk; jj = -jj; }
+        if (dxy < 0.0) { const double tt = jj; jj = -kk; kk = tt; }
+        double s0, s1;
+        if (a.use_covariance_for_scale) {
+          // extractScaleOfCovariance (shaders.wgsl:240-246)
+          double cov[3][3];
+          covariance(Tv, pos, u, cov);
+          const double cd = cov[0][0] - cov[1][1];
+          const double b = sqrt(cd * cd + (4.0 * cov[0][1]) * cov[0][1]);
+          s0 = sqrt(((cov[0][0] + cov[1][1]) + b) * 0.5);
+          s1 = sqrt(((cov[0][0] + cov[1][1]) - b) * 0.5);
+        } else {
+          // extractScaleOfEllipse (shaders.wgsl:217-237)
+          const double d = ((2.0 * M[0][1]) * jj) * kk;
+          const double e = M[2][2] - (((M[0][0] * t0) * t0 + (M[1][1] * t1) * t1) + ((2.0 * M[0][1]) * t0) * t1);
+          s0 = sqrt(fabs(e / (((M[0][0] * kk) * kk + (M[1][1] * jj) * jj) - d)));
+          s1 = sqrt(fabs(e / (((M[0][0] * jj) * jj + (M[1][1] * kk) * kk) + d)));
+        }
+        const double bias = (double)u.ellipse_size_bias;
+        const double sa0 = bias + s0, sb0 = bias + s1;
+        const float A00 = (float)(kk * sa0), A10 = (float)(-jj * sa0);
+        const float A01 = (float)(jj * sb0), A11 = (float)(kk * sb0);
+        const float tx = (float)t0, ty = (float)t1;
+        uint32_t box_lo, box_hi;
+        pixel_box(A00, A10, A01, A11, tx, ty, u, a.use_unaligned_rectangles != 0, box_lo, box_hi);
+        float4* rec = a.records + v * a.rec_stride + 3 * (size_t)c;
+        rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
+        rec[1] = make_float4(A00, A10, A01, A11);
+        rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
+        if (a.conics && v == 0) {
+          double* cn = a.conics + 6 * (size_t)c;
+          cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];
+          cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 

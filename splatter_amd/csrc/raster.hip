@@ -62,17 +62,24 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   __shared__ uint64_t s_prefix;
 
   const uint32_t tid = threadIdx.x;
+  const uint32_t v = blockIdx.y;
+  uint32_t* counters = a.counters + (size_t)v * kCounterWords;
+  const uint32_t* order = a.order ? a.order + v * a.order_stride : nullptr;
+  const float4* records = a.records + v * a.rec_stride;
+  uint32_t* pair_keys = a.pair_keys + v * a.pair_stride;
+  uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
+  uint64_t* status = a.status + v * a.status_stride;
   for (int i = tid; i < a.pair_passes * 256; i += THREADS) s_hist[i] = 0u;
-  const uint32_t n = *a.count;
+  const uint32_t n = counters[kVisible];
   const uint32_t num_tiles = (n + TILE - 1) / TILE;
   if (num_tiles == 0 && blockIdx.x == 0 && tid == 0) {
-    a.counters[kPairs] = 0u;
-    a.counters[kPairOverflow] = 0u;
+    counters[kPairs] = 0u;
+    counters[kPairOverflow] = 0u;
   }
-  const uint32_t tiles_x = a.tiles_x;
+  const uint32_t tiles_x = a.uniforms[v].tiles_x;
 
   while (true) {
-    if (tid == 0) s_tile = atomicAdd(&a.counters[kCtrBin], 1u);
+    if (tid == 0) s_tile = atomicAdd(&counters[kCtrBin], 1u);
     __syncthreads();
     const uint32_t t = s_tile;
     if (t >= num_tiles) break;
@@ -83,8 +90,8 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
       const uint32_t r = t * TILE + slot;
       cnt[k] = 0;
       if (r < n) {
-        const uint32_t c = a.order ? a.order[r] : r;
-        const float4 q = a.records[3 * (size_t)c + 2];
+        const uint32_t c = order ? order[r] : r;
+        const float4 q = records[3 * (size_t)c + 2];
         const uint32_t lo = __float_as_uint(q.z), hi = __float_as_uint(q.w);
         const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;
         const uint32_t x1 = (hi & 0xFFFFu) / kTileSize, y1 = (hi >> 16) / kTileSize;
@@ -105,15 +112,15 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
       excl += cnt[k];
     }
     if (tid < kWave) {
-      const uint64_t prefix = wave_lookback64(a.status, t, (uint64_t)total);
+      const uint64_t prefix = wave_lookback64(status, t, (uint64_t)total);
       if (tid == 0) {
         s_off[TILE] = total;
         s_prefix = prefix;
       }
       if (tid == 0 && t == num_tiles - 1) {
         const uint64_t k_all = prefix + total;
-        a.counters[kPairs] = (uint32_t)(k_all < a.pair_cap ? k_all : a.pair_cap);
-        a.counters[kPairOverflow] = k_all > a.pair_cap ? 1u : 0u;
+        counters[kPairs] = (uint32_t)(k_all < a.pair_cap ? k_all : a.pair_cap);
+        counters[kPairOverflow] = k_all > a.pair_cap ? 1u : 0u;
       }
     }
     __syncthreads();
@@ -129,28 +136,32 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
         const uint32_t k = s - s_off[lo];
         const uint32_t w = s_w[lo], org = s_org[lo];
         const uint32_t tile = ((org >> 16) + k / w) * tiles_x + (org & 0xFFFFu) + k % w;
-        a.pair_keys[pos] = tile;
-        a.pair_vals[pos] = s_rec[lo];
+        pair_keys[pos] = tile;
+        pair_vals[pos] = s_rec[lo];
         for (int q = 0; q < a.pair_passes; ++q) atomicAdd(&s_hist[q * 256 + ((tile >> (8 * q)) & 255u)], 1u);
       }
     }
     __syncthreads();
   }
   for (int i = tid; i < a.pair_passes * 256; i += THREADS)
-    if (s_hist[i]) atomicAdd(&a.hist[i], s_hist[i]);
-  finish_persistent(&a.counters[kCtrBin], &a.counters[kDone + kCtrBin]);
+    if (s_hist[i]) atomicAdd(&a.hist[v * a.hist_stride + i], s_hist[i]);
+  finish_persistent(&counters[kCtrBin], &counters[kDone + kCtrBin]);
 }
 
 __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
-  clear_words(a.clear_a, a.clear_a_words);
-  clear_words(a.clear_b, a.clear_b_words);
-  uint32_t n = a.counters[kPairs];
+  const uint32_t v = blockIdx.y;
+  if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
+  if (a.clear_b) clear_words(a.clear_b + v * a.clear_b_stride, a.clear_b_words);
+  const uint32_t* keys = a.pair_keys + v * a.pair_stride;
+  uint2* ranges = a.ranges + v * a.ranges_stride;
+  const uint32_t tiles = a.uniforms[v].tiles_x * a.uniforms[v].tiles_y;
+  uint32_t n = a.counters[(size_t)v * kCounterWords + kPairs];
   if (n > a.pair_cap) n = a.pair_cap;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t t = a.pair_keys[i];
-    if (t >= a.tiles) continue;
-    if (i == 0 || a.pair_keys[i - 1] != t) a.ranges[t].x = i;
-    if (i == n - 1 || a.pair_keys[i + 1] != t) a.ranges[t].y = i + 1;
+    const uint32_t t = keys[i];
+    if (t >= tiles) continue;
+    if (i == 0 || keys[i - 1] != t) ranges[t].x = i;
+    if (i == n - 1 || keys[i + 1] != t) ranges[t].y = i + 1;
   }
 }
 
@@ -170,8 +181,12 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   __shared__ uint2 s_box[2][B];    // pixel box lo / hi (x | y << 16)
   __shared__ __align__(16) uint8_t s_list[4][B];  // per-wave compacted entry slots, read 4 at a time
 
-  const Uniforms& u = a.u;
+  const uint32_t v = blockIdx.y;
+  const Uniforms& u = a.uniforms[v];
   const uint32_t T = u.tiles_x * u.tiles_y;
+  uint2* ranges = a.ranges + v * a.ranges_stride;
+  const uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
+  const float4* records = a.records + v * a.rec_stride;
   // XCD-aware order: blocks are dealt round-robin to the 8 XCDs, so give
   // each XCD a contiguous band of tiles (neighbouring tiles share records).
   const uint32_t per_xcd = gridDim.x / 8;
@@ -183,9 +198,9 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   const uint32_t px = qx0 + (lane & 7u), py = qy0 + (lane >> 3);
   const uint32_t W = u.image_size[0], H = u.image_size[1];
   const bool inside = px < W && py < H;
-  const uint2 range = a.ranges[tile];
+  const uint2 range = ranges[tile];
   __syncthreads();
-  if (tid == 0) a.ranges[tile] = make_uint2(0u, 0u);
+  if (tid == 0) ranges[tile] = make_uint2(0u, 0u);
 
   const float qx = ((((float)px + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
   const float qy = (1.0f - (((float)py + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
@@ -193,17 +208,17 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   const bool unaligned = a.use_unaligned_rectangles != 0;
   const float margin = u.ellipse_margin;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = f32 ? 1.0f : 255.0f;
-  const uint32_t visible = *a.visible;
+  const uint32_t visible = a.counters[(size_t)v * kCounterWords + kVisible];
 
   // register stage of one entry per lane
   float4 g0 = make_float4(0, 0, 0, 0), g1 = g0, g2 = g0;
   auto fetch = [&](uint32_t base) {
     if (base + tid < range.y) {
-      uint32_t c = a.pair_vals[base + tid];
+      uint32_t c = pair_vals[base + tid];
       if (c >= visible) c = 0;  // only after an upstream error; never read past the records
-      g0 = a.records[3 * (size_t)c + 0];
-      g1 = a.records[3 * (size_t)c + 1];
-      g2 = a.records[3 * (size_t)c + 2];
+      g0 = records[3 * (size_t)c + 0];
+      g1 = records[3 * (size_t)c + 1];
+      g2 = records[3 * (size_t)c + 2];
     }
   };
   auto stage = [&](int buf) {
@@ -301,7 +316,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     buf ^= 1;
     if (__syncthreads_and(wave_done)) break;
   }
-  if (a.tile_work && tid == 0) a.tile_work[tile] = make_uint2(range.y - range.x, consumed);
+  if (a.tile_work && v == 0 && tid == 0) a.tile_work[tile] = make_uint2(range.y - range.x, consumed);
   if (inside) {
     uint32_t r, g, b, al;
     if (!f32) {
@@ -309,26 +324,21 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     } else {
       r = (uint32_t)q8(c0); g = (uint32_t)q8(c1); b = (uint32_t)q8(c2); al = (uint32_t)q8(tr);
     }
-    a.pixels[(size_t)py * W + px] = a.bgra ? (b | (g << 8) | (r << 16) | (al << 24))
+    a.pixels[v][(size_t)py * W + px] = a.bgra ? (b | (g << 8) | (r << 16) | (al << 24))
                                             : (r | (g << 8) | (b << 16) | (al << 24));
   }
 }
 
 void launch_bin(const BinLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_bin<kBinThreads, kBinItems>), dim3(a.grid), dim3(kBinThreads), 0, s, a);
+  hipLaunchKernelGGL((k_bin<kBinThreads, kBinItems>), dim3(a.grid_x, a.views), dim3(kBinThreads), 0, s, a);
 }
 
-void launch_ranges(const RangesLaunch& a, uint32_t max_pairs, hipStream_t s) {
-  uint32_t grid = (max_pairs + 255) / 256;
-  if (grid > 4096) grid = 4096;
-  if (grid == 0) grid = 1;
-  hipLaunchKernelGGL(k_tile_ranges, dim3(grid), dim3(256), 0, s, a);
+void launch_ranges(const RangesLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_ranges, dim3(a.grid_x, a.views), dim3(256), 0, s, a);
 }
 
 void launch_raster(const RasterLaunch& a, hipStream_t s) {
-  const uint32_t T = a.u.tiles_x * a.u.tiles_y;
-  const uint32_t grid = ((T + 7) / 8) * 8;
-  hipLaunchKernelGGL(k_raster, dim3(grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_raster, dim3(a.grid_x, a.views), dim3(256), 0, s, a);
 }
 
 }  // namespace spl

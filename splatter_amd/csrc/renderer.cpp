@@ -1,0 +1,788 @@
+// renderer.cpp -- Renderer (renderer.rs:64-467): workspace, the per-frame host
+// math of renderer.rs:364-409, and the launch sequence of the splat pipeline
+// for a batch of views, replayed from a hipGraph per batch shape.  Built with
+// -ffp-contract=off so the host-side f32 camera math rounds as the
+// reference's Rust does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "spl_internal.h"
+#include "spl_objects.h"
+#include "splatter.h"
+
+using namespace spl;
+
+namespace {
+
+constexpr int kStageSlots = 4;  // pinned uniform uploads in flight
+constexpr int kStageCount = 8;  // cull, histograms, sort, project, bin, pair sort, ranges, raster
+
+struct UploadSlot {
+  Uniforms* uniforms = nullptr;  // pinned host
+  uint32_t** pixels = nullptr;   // pinned host
+  hipEvent_t done = nullptr;
+};
+
+// Everything a captured launch sequence bakes into its kernel arguments.
+struct BatchShape {
+  const spl_scene* scene = nullptr;
+  uint64_t n = 0;
+  uint32_t views = 0;
+  uint32_t max_tiles = 0;
+  int pair_passes = 0;
+  bool conics = false;
+  bool tile_stats = false;
+  bool operator==(const BatchShape& o) const {
+    return scene == o.scene && n == o.n && views == o.views && max_tiles == o.max_tiles &&
+           pair_passes == o.pair_passes && conics == o.conics && tile_stats == o.tile_stats;
+  }
+};
+
+int pair_passes_for(uint32_t tiles) {
+  if (tiles <= 256) return 1;
+  if (tiles <= 65536) return 2;
+  return 3;
+}
+
+}  // namespace
+
+struct spl_renderer {
+  spl_device* dev = nullptr;
+  spl_configuration cfg{};
+  uint64_t cap = 0;
+  uint32_t max_views = 1;
+  int depth_bits = 8, depth_passes = 4;
+  uint64_t pair_cap = 0;
+  // per-view sizes (words)
+  size_t cull_tiles = 0, sort_words = 0, bin_words = 0, pair_words = 0;
+  // per-view buffers, view v at base + v * stride
+  uint32_t* keys[2] = {nullptr, nullptr};
+  uint32_t* vals[2] = {nullptr, nullptr};
+  uint32_t* compact_idx = nullptr;
+  float4* records = nullptr;
+  uint32_t* pair_keys[2] = {nullptr, nullptr};
+  uint32_t* pair_vals[2] = {nullptr, nullptr};
+  uint32_t* cull_status = nullptr;
+  uint32_t* tile_prefix = nullptr;
+  uint32_t* sort_status = nullptr;
+  uint64_t* bin_status = nullptr;
+  uint32_t* pair_status = nullptr;
+  uint32_t* counters = nullptr;
+  uint32_t* depth_hist = nullptr;
+  uint32_t* pair_hist = nullptr;
+  uint2* ranges = nullptr;
+  uint32_t ranges_cap = 0;
+  // shared
+  uint32_t* vis = nullptr;
+  Uniforms* d_uniforms = nullptr;
+  uint32_t** d_pixels = nullptr;
+  UploadSlot slots[kStageSlots];
+  int next_slot = 0;
+  double* conics = nullptr;
+  bool conics_on = false;
+  uint2* tile_work = nullptr;
+  uint32_t tile_work_cap = 0;
+  bool tile_stats_on = false;
+  // last batch
+  uint32_t last_views = 0;
+  bool last_sorted = false;
+  int last_final = 0;
+  bool have_frame = false;
+  uint32_t debug_view = 0;
+  uint32_t last_tiles[kMaxViews] = {};
+  // stage timing: one set of events per batch, read out in bulk
+  bool timing = false;
+  std::vector<std::array<hipEvent_t, kStageCount + 1>> ev_pool;
+  std::vector<uint32_t> ev_views;
+  size_t ev_used = 0;
+  double stage_sum_ms[kStageCount] = {};
+  uint64_t stage_frames = 0;
+  // replay of the launch sequence
+  hipGraphExec_t graph = nullptr;
+  BatchShape graph_shape;
+};
+
+namespace spl {
+spl_device* renderer_device(const spl_renderer* r) { return r->dev; }
+uint64_t renderer_capacity(const spl_renderer* r) { return r->cap; }
+}  // namespace spl
+
+namespace {
+
+void drop_graph(spl_renderer* r) {
+  if (r->graph) (void)hipGraphExecDestroy(r->graph);
+  r->graph = nullptr;
+  r->graph_shape = BatchShape{};
+}
+
+void renderer_free(spl_renderer* r) {
+  drop_graph(r);
+  for (int i = 0; i < 2; ++i) {
+    dev_free(r->keys[i]);
+    dev_free(r->vals[i]);
+    dev_free(r->pair_keys[i]);
+    dev_free(r->pair_vals[i]);
+  }
+  dev_free(r->compact_idx);
+  dev_free(r->records);
+  dev_free(r->cull_status);
+  dev_free(r->tile_prefix);
+  dev_free(r->sort_status);
+  dev_free(r->bin_status);
+  dev_free(r->pair_status);
+  dev_free(r->counters);
+  dev_free(r->depth_hist);
+  dev_free(r->pair_hist);
+  dev_free(r->ranges);
+  dev_free(r->vis);
+  dev_free(r->d_uniforms);
+  dev_free(r->d_pixels);
+  dev_free(r->conics);
+  dev_free(r->tile_work);
+  for (auto& s : r->slots) {
+    if (s.uniforms) (void)hipHostFree(s.uniforms);
+    if (s.pixels) (void)hipHostFree(s.pixels);
+    if (s.done) (void)hipEventDestroy(s.done);
+    s = UploadSlot{};
+  }
+  for (auto& set : r->ev_pool)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e), e = nullptr;
+  r->ev_pool.clear();
+  r->ev_views.clear();
+  r->ev_used = 0;
+}
+
+// utils.rs:58-65
+void mat4_multiplication(const float a[16], const float b[16], float out[16]) {
+  float t[16];
+  for (int j = 0; j < 4; ++j)
+    for (int r = 0; r < 4; ++r)
+      t[4 * j + r] = ((a[r] * b[4 * j + 0] + a[4 + r] * b[4 * j + 1]) + a[8 + r] * b[4 * j + 2]) + a[12 + r] * b[4 * j + 3];
+  std::memcpy(out, t, sizeof(t));
+}
+
+// renderer.rs:364-370,397-409 and utils.rs:45-55.  field_of_view_y replaces
+// the hard-coded PI/2 (renderer.rs:366) and view_width = aspect*view_height
+// replaces aspect/view_height (renderer.rs:368): at 90 degrees tanf gives
+// exactly 1.0f and both are bit-identical.
+void frame_uniforms(const spl_configuration& cfg, const float cam[16], const float view[16], uint32_t W, uint32_t H,
+                    Uniforms& u) {
+  const float view_height = tanf(cfg.field_of_view_y * 0.5f);
+  const float view_width = ((float)W / (float)H) * view_height;
+  float P[16] = {};
+  const float near_ = 1.0f, far_ = 1000.0f;
+  const float denominator = 1.0f / (near_ - far_);
+  P[0] = 1.0f / view_width;
+  P[5] = 1.0f / view_height;
+  P[10] = -far_ * denominator;
+  P[11] = 1.0f;
+  P[14] = (near_ * far_) * denominator;
+  std::memcpy(u.camera_matrix, cam, 64);
+  std::memcpy(u.view_matrix, view, 64);
+  mat4_multiplication(P, view, u.view_projection_matrix);
+  u.view_size[0] = view_width;
+  u.view_size[1] = view_height;
+  u.image_size[0] = W;
+  u.image_size[1] = H;
+  u.frustum_culling_tolerance = cfg.frustum_culling_tolerance;
+  u.ellipse_size_bias = (0.2f * view_width) / (float)W;
+  u.ellipse_margin = cfg.ellipse_margin;
+  u.splat_scale = cfg.splat_scale;
+  u.tiles_x = div_up(W, kTileSize);
+  u.tiles_y = div_up(H, kTileSize);
+}
+
+int ensure_ranges(spl_renderer* r, uint32_t tiles) {
+  if (tiles <= r->ranges_cap) return SPL_OK;
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  drop_graph(r);
+  dev_free(r->ranges);
+  r->ranges_cap = 0;
+  const size_t words = (size_t)tiles * r->max_views;
+  HIP_TRY(dev_alloc(&r->ranges, words));
+  HIP_TRY(hipMemset(r->ranges, 0, words * sizeof(uint2)));
+  HIP_TRY(hipDeviceSynchronize());
+  r->ranges_cap = tiles;
+  return SPL_OK;
+}
+
+int ensure_tile_work(spl_renderer* r, uint32_t tiles) {
+  if (!r->tile_stats_on || r->tile_work_cap >= tiles) return SPL_OK;
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  drop_graph(r);
+  dev_free(r->tile_work);
+  r->tile_work_cap = 0;
+  HIP_TRY(dev_alloc(&r->tile_work, tiles));
+  r->tile_work_cap = tiles;
+  return SPL_OK;
+}
+
+// The whole pipeline for one batch, as a sequence of launches on `q`.
+// Every stage clears the bookkeeping of an earlier stage once that stage is
+// done (status words, histograms, tile ranges), so the sequence needs no
+// memset and replays as a graph.
+void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEvent_t* ev) {
+  const spl_configuration& cfg = r->cfg;
+  const spl_scene* s = b.scene;
+  const uint32_t n = (uint32_t)b.n, nv = b.views;
+  const bool sorted = cfg.depth_sorting != SPL_DEPTH_SORTING_NONE;
+  const int P = sorted ? r->depth_passes : 0;
+  const int bits = r->depth_bits;
+  const int radix = 1 << bits;
+  const int Q = b.pair_passes;
+  const uint32_t cus = (uint32_t)r->dev->cu_count;
+  const uint32_t per_view = std::max<uint32_t>(1, cus * 4 / nv);
+  const size_t cap = r->cap;
+  const size_t sort_stride = r->sort_words * kMaxDepthPasses;
+  const size_t pair_stride_status = r->pair_words * kMaxPairPasses;
+  const size_t sort_used = (size_t)div_up(n, kSortThreads * kSortItems) * radix;
+  const size_t cull_used = div_up(n, kCullTile);
+  auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], q); };
+  auto clampg = [](uint64_t want, uint32_t hi) { return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, hi)); };
+
+  mark(0);
+  {
+    CullLaunch a{};
+    a.centers = s->centers;
+    a.n = n;
+    a.views = nv;
+    a.uniforms = r->d_uniforms;
+    a.gpu_predicate = sorted ? 0 : 1;
+    a.write_keys = sorted ? 1 : 0;
+    a.keys = r->keys[0]; a.keys_stride = cap;
+    a.idx = r->compact_idx; a.idx_stride = cap;
+    a.vis = r->vis;
+    a.tile_prefix = r->tile_prefix; a.tp_stride = r->cull_tiles;
+    a.status = r->cull_status; a.status_stride = r->cull_tiles;
+    a.counters = r->counters;
+    a.grid = clampg(div_up(n, kCullTile), cus * 4);
+    launch_cull(a, q);
+  }
+  mark(1);
+  if (sorted) {
+    HistLaunch a{};
+    a.keys = r->keys[0]; a.keys_stride = cap;
+    a.hist = r->depth_hist; a.hist_stride = (size_t)kMaxDepthPasses * 1024;
+    a.counters = r->counters;
+    a.passes = P;
+    a.bits = bits;
+    a.clear = r->cull_status; a.clear_words = cull_used; a.clear_stride = r->cull_tiles;
+    a.grid_x = clampg(div_up(n, 256 * 16), per_view);
+    a.views = (int)nv;
+    launch_histograms(a, q);
+  }
+  mark(2);
+  for (int p = 0; p < P; ++p) {
+    SortPassLaunch a{};
+    a.keys_in = r->keys[p & 1];
+    a.vals_in = p == 0 ? nullptr : r->vals[p & 1];
+    a.keys_out = r->keys[(p + 1) & 1];
+    a.vals_out = r->vals[(p + 1) & 1];
+    a.entry_stride = cap;
+    a.counters = r->counters;
+    a.count_slot = kVisible;
+    a.count_cap = n;
+    a.shift = p * bits;
+    a.hist = r->depth_hist + (size_t)p * radix; a.hist_stride = (size_t)kMaxDepthPasses * 1024;
+    a.status = r->sort_status + (size_t)p * r->sort_words; a.status_stride = sort_stride;
+    a.tile_slot = kCtrSort0 + p;
+    if (p > 0) {
+      a.clear_a = r->sort_status + (size_t)(p - 1) * r->sort_words;
+      a.clear_a_words = sort_used;
+      a.clear_a_stride = sort_stride;
+    }
+    a.grid_x = clampg(div_up(n, kSortThreads * kSortItems), per_view);
+    a.views = (int)nv;
+    launch_sort_pass(bits, a, q);
+  }
+  const int fin = P & 1;
+  mark(3);
+  {
+    ProjectLaunch a{};
+    a.scene = SceneView{s->centers, s->rotations, s->scale_alpha, s->sh, n};
+    a.views = nv;
+    a.uniforms = r->d_uniforms;
+    a.vis = r->vis;
+    a.tile_prefix = r->tile_prefix; a.tp_stride = r->cull_tiles;
+    a.sh_order = (int)cfg.spherical_harmonics_order;
+    a.use_covariance_for_scale = cfg.use_covariance_for_scale ? 1 : 0;
+    a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
+    a.records = r->records; a.rec_stride = cap * kRecordFloat4;
+    a.conics = b.conics ? r->conics : nullptr;
+    if (P > 0) {
+      a.clear_a = r->sort_status + (size_t)(P - 1) * r->sort_words;
+      a.clear_a_words = sort_used;
+      a.clear_a_stride = sort_stride;
+      a.clear_b = r->depth_hist;
+      a.clear_b_words = (size_t)kMaxDepthPasses * 1024;
+      a.clear_b_stride = (size_t)kMaxDepthPasses * 1024;
+    } else {
+      a.clear_a = r->cull_status;
+      a.clear_a_words = cull_used;
+      a.clear_a_stride = r->cull_tiles;
+    }
+    a.grid = clampg(div_up(n, kCullTile), cus * 4);
+    launch_project(a, q);
+  }
+  mark(4);
+  {
+    BinLaunch a{};
+    a.order = sorted ? r->vals[fin] : nullptr; a.order_stride = cap;
+    a.records = r->records; a.rec_stride = cap * kRecordFloat4;
+    a.uniforms = r->d_uniforms;
+    a.pair_keys = r->pair_keys[0];
+    a.pair_vals = r->pair_vals[0]; a.pair_stride = r->pair_cap;
+    a.pair_cap = (uint32_t)r->pair_cap;
+    a.hist = r->pair_hist; a.hist_stride = (size_t)kMaxPairPasses * 256;
+    a.pair_passes = Q;
+    a.status = r->bin_status; a.status_stride = r->bin_words;
+    a.counters = r->counters;
+    a.grid_x = clampg(div_up(n, kBinThreads * kBinItems), per_view);
+    a.views = (int)nv;
+    launch_bin(a, q);
+  }
+  mark(5);
+  for (int p = 0; p < Q; ++p) {
+    SortPassLaunch a{};
+    a.keys_in = r->pair_keys[p & 1];
+    a.vals_in = r->pair_vals[p & 1];
+    a.keys_out = r->pair_keys[(p + 1) & 1];
+    a.vals_out = r->pair_vals[(p + 1) & 1];
+    a.entry_stride = r->pair_cap;
+    a.counters = r->counters;
+    a.count_slot = kPairs;
+    a.count_cap = (uint32_t)r->pair_cap;
+    a.shift = 8 * p;
+    a.hist = r->pair_hist + (size_t)p * 256; a.hist_stride = (size_t)kMaxPairPasses * 256;
+    a.status = r->pair_status + (size_t)p * r->pair_words; a.status_stride = pair_stride_status;
+    a.tile_slot = kCtrPair0 + p;
+    if (p == 0) {
+      a.clear_a = reinterpret_cast<uint32_t*>(r->bin_status);
+      a.clear_a_words = (size_t)div_up(n, kBinThreads * kBinItems) * 2;
+      a.clear_a_stride = r->bin_words * 2;
+    } else {
+      a.clear_a = r->pair_status + (size_t)(p - 1) * r->pair_words;
+      a.clear_a_words = r->pair_words;
+      a.clear_a_stride = pair_stride_status;
+    }
+    a.grid_x = clampg(div_up(r->pair_cap, kSortThreads * kSortItems), per_view);
+    a.views = (int)nv;
+    launch_sort_pass(8, a, q);
+  }
+  const int pfin = Q & 1;
+  mark(6);
+  {
+    RangesLaunch a{};
+    a.pair_keys = r->pair_keys[pfin]; a.pair_stride = r->pair_cap;
+    a.counters = r->counters;
+    a.pair_cap = (uint32_t)r->pair_cap;
+    a.uniforms = r->d_uniforms;
+    a.ranges = r->ranges; a.ranges_stride = r->ranges_cap;
+    a.clear_a = r->pair_status + (size_t)(Q - 1) * r->pair_words;
+    a.clear_a_words = r->pair_words;
+    a.clear_a_stride = pair_stride_status;
+    a.clear_b = r->pair_hist;
+    a.clear_b_words = (size_t)kMaxPairPasses * 256;
+    a.clear_b_stride = (size_t)kMaxPairPasses * 256;
+    a.grid_x = clampg(div_up(r->pair_cap, 256), std::max<uint32_t>(1, cus * 8 / nv));
+    a.views = (int)nv;
+    launch_ranges(a, q);
+  }
+  mark(7);
+  {
+    RasterLaunch a{};
+    a.pair_vals = r->pair_vals[pfin]; a.pair_stride = r->pair_cap;
+    a.records = r->records; a.rec_stride = cap * kRecordFloat4;
+    a.ranges = r->ranges; a.ranges_stride = r->ranges_cap;
+    a.uniforms = r->d_uniforms;
+    a.pixels = r->d_pixels;
+    a.counters = r->counters;
+    a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
+    a.blend_f32 = cfg.blend_mode == SPL_BLEND_F32 ? 1 : 0;
+    a.bgra = cfg.frame_format == SPL_FRAME_FORMAT_BGRA8_UNORM ? 1 : 0;
+    a.tile_work = b.tile_stats ? r->tile_work : nullptr;
+    a.grid_x = (int)(((b.max_tiles + 7) / 8) * 8);
+    a.views = (int)nv;
+    launch_raster(a, q);
+  }
+  mark(8);
+}
+
+// One batch of at most max_views views.
+int enqueue_batch(spl_renderer* r, spl_scene* s, uint32_t nv, spl_frame* const* frames, const float* cams,
+                  const float* views) {
+  hipStream_t q = r->dev->stream;
+  Uniforms u[kMaxViews];
+  uint32_t max_tiles = 1;
+  for (uint32_t v = 0; v < nv; ++v) {
+    spl_frame* f = frames[v];
+    if (!f) return fail(SPL_ERROR_INVALID_ARGUMENT, "null frame");
+    if (f->dev != r->dev) return fail(SPL_ERROR_INVALID_ARGUMENT, "frame and renderer must share a device");
+    frame_uniforms(r->cfg, cams + 16 * (size_t)v, views + 16 * (size_t)v, f->width, f->height, u[v]);
+    max_tiles = std::max(max_tiles, u[v].tiles_x * u[v].tiles_y);
+    r->last_tiles[v] = u[v].tiles_x * u[v].tiles_y;
+  }
+  int rc = ensure_ranges(r, max_tiles);
+  if (rc) return rc;
+  rc = ensure_tile_work(r, max_tiles);
+  if (rc) return rc;
+
+  // stage this batch's uniforms and frame pointers through pinned memory
+  UploadSlot& slot = r->slots[r->next_slot];
+  r->next_slot = (r->next_slot + 1) % kStageSlots;
+  HIP_TRY(hipEventSynchronize(slot.done));
+  std::memcpy(slot.uniforms, u, nv * sizeof(Uniforms));
+  for (uint32_t v = 0; v < nv; ++v) slot.pixels[v] = frames[v]->pixels;
+  HIP_TRY(hipMemcpyAsync(r->d_uniforms, slot.uniforms, nv * sizeof(Uniforms), hipMemcpyHostToDevice, q));
+  HIP_TRY(hipMemcpyAsync(r->d_pixels, slot.pixels, nv * sizeof(uint32_t*), hipMemcpyHostToDevice, q));
+  HIP_TRY(hipEventRecord(slot.done, q));
+
+  BatchShape b;
+  b.scene = s;
+  b.n = s->count;
+  b.views = nv;
+  b.max_tiles = max_tiles;
+  b.pair_passes = pair_passes_for(max_tiles);
+  b.conics = r->conics_on;
+  b.tile_stats = r->tile_stats_on;
+
+  if (r->timing) {
+    if (r->ev_used == r->ev_pool.size()) {
+      std::array<hipEvent_t, kStageCount + 1> set{};
+      for (auto& e : set) HIP_TRY(hipEventCreate(&e));
+      r->ev_pool.push_back(set);
+      r->ev_views.push_back(0);
+    }
+    r->ev_views[r->ev_used] = nv;
+    launch_sequence(r, b, q, r->ev_pool[r->ev_used++].data());
+  } else if (r->cfg.use_hip_graphs) {
+    if (!r->graph || !(r->graph_shape == b)) {
+      drop_graph(r);
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
+      launch_sequence(r, b, q, nullptr);
+      hipError_t e = hipStreamEndCapture(q, &g);
+      if (e != hipSuccess) return fail(SPL_ERROR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        r->graph = nullptr;
+        return fail(SPL_ERROR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+      }
+      r->graph_shape = b;
+    }
+    HIP_TRY(hipGraphLaunch(r->graph, q));
+  } else {
+    launch_sequence(r, b, q, nullptr);
+  }
+  HIP_TRY(hipGetLastError());
+  r->last_views = nv;
+  r->last_sorted = r->cfg.depth_sorting != SPL_DEPTH_SORTING_NONE;
+  r->last_final = r->last_sorted ? (r->depth_passes & 1) : 0;
+  r->have_frame = true;
+  if (r->debug_view >= nv) r->debug_view = 0;
+  return SPL_OK;
+}
+
+int read_counter(spl_renderer* r, uint32_t view, int slot, uint32_t* v) {
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  HIP_TRY(hipMemcpy(v, r->counters + (size_t)view * kCounterWords + slot, 4, hipMemcpyDeviceToHost));
+  return SPL_OK;
+}
+
+// draw order of the selected view: the compacted position of each drawn splat
+int draw_order(spl_renderer* r, std::vector<uint32_t>& order, std::vector<uint32_t>* keys) {
+  const uint32_t view = r->debug_view;
+  uint32_t v = 0, err = 0;
+  int rc = read_counter(r, view, kScatterError, &err);
+  if (rc) return rc;
+  if (err) return fail(SPL_ERROR_DEVICE, "radix sort scatter left its range (inconsistent histogram)");
+  rc = read_counter(r, view, kVisible, &v);
+  if (rc) return rc;
+  order.resize(v);
+  if (keys) keys->assign(v, 0u);
+  if (v == 0) return SPL_OK;
+  const size_t off = (size_t)view * r->cap;
+  if (r->last_sorted) {
+    HIP_TRY(hipMemcpy(order.data(), r->vals[r->last_final] + off, (size_t)v * 4, hipMemcpyDeviceToHost));
+    if (keys) HIP_TRY(hipMemcpy(keys->data(), r->keys[r->last_final] + off, (size_t)v * 4, hipMemcpyDeviceToHost));
+  } else {
+    for (uint32_t i = 0; i < v; ++i) order[i] = i;
+  }
+  return SPL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_renderer** out) {
+  if (!dev || !cfg || !out) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_create: null argument");
+  *out = nullptr;
+  if (cfg->spherical_harmonics_order > 3)
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "spherical_harmonics_order must be 0..=3");
+  if (cfg->radix_bits_per_digit != 8 && cfg->radix_bits_per_digit != 10)
+    return fail(SPL_ERROR_UNSUPPORTED, "radix_bits_per_digit must be 8 or 10");
+  if (cfg->max_splat_count == 0 || cfg->max_splat_count > (1ull << 30))
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "max_splat_count must be in 1..=2^30");
+  if ((unsigned)cfg->depth_sorting > 3u || (unsigned)cfg->frame_format > 1u || (unsigned)cfg->blend_mode > 1u)
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_create: enum out of range");
+  if (!(cfg->field_of_view_y > 0.0f && cfg->field_of_view_y < 3.14159f))
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "field_of_view_y must be in (0, pi)");
+  if (cfg->max_views > (uint32_t)kMaxViews)
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "max_views must be in 1..=32");
+  HIP_TRY(hipSetDevice(dev->ordinal));
+  auto* r = new (std::nothrow) spl_renderer();
+  if (!r) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: host allocation");
+  r->dev = dev;
+  r->cfg = *cfg;
+  if (r->cfg.tile_pairs_per_splat == 0) r->cfg.tile_pairs_per_splat = 4;
+  if (r->cfg.max_views == 0) r->cfg.max_views = 1;
+  r->max_views = r->cfg.max_views;
+  r->cap = cfg->max_splat_count;
+  r->depth_bits = (int)cfg->radix_bits_per_digit;
+  r->depth_passes = (30 + r->depth_bits - 1) / r->depth_bits;
+  const int radix = 1 << r->depth_bits;
+  r->pair_cap = std::min<uint64_t>(r->cap * (uint64_t)r->cfg.tile_pairs_per_splat, (1ull << 30) - (1ull << 20));
+  r->cull_tiles = div_up(r->cap, kCullTile);
+  r->sort_words = (size_t)div_up(r->cap, kSortThreads * kSortItems) * radix;
+  r->bin_words = div_up(r->cap, kBinThreads * kBinItems);
+  r->pair_words = (size_t)div_up(r->pair_cap, kSortThreads * kSortItems) * 256;
+  const size_t V = r->max_views;
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  for (int i = 0; i < 2; ++i) {
+    chk(dev_alloc(&r->keys[i], V * r->cap));
+    chk(dev_alloc(&r->vals[i], V * r->cap));
+    chk(dev_alloc(&r->pair_keys[i], V * r->pair_cap));
+    chk(dev_alloc(&r->pair_vals[i], V * r->pair_cap));
+  }
+  chk(dev_alloc(&r->compact_idx, V * r->cap));
+  chk(dev_alloc(&r->records, V * r->cap * kRecordFloat4));
+  chk(dev_alloc(&r->cull_status, V * r->cull_tiles));
+  chk(dev_alloc(&r->tile_prefix, V * r->cull_tiles));
+  chk(dev_alloc(&r->sort_status, V * r->sort_words * kMaxDepthPasses));
+  chk(dev_alloc(&r->bin_status, V * r->bin_words));
+  chk(dev_alloc(&r->pair_status, V * r->pair_words * kMaxPairPasses));
+  chk(dev_alloc(&r->counters, V * kCounterWords));
+  chk(dev_alloc(&r->depth_hist, V * kMaxDepthPasses * 1024));
+  chk(dev_alloc(&r->pair_hist, V * kMaxPairPasses * 256));
+  chk(dev_alloc(&r->vis, r->cap));
+  chk(dev_alloc(&r->d_uniforms, V));
+  chk(dev_alloc(&r->d_pixels, V));
+  for (auto& sl : r->slots) {
+    chk(hipHostMalloc((void**)&sl.uniforms, V * sizeof(Uniforms), hipHostMallocDefault));
+    chk(hipHostMalloc((void**)&sl.pixels, V * sizeof(uint32_t*), hipHostMallocDefault));
+    chk(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  }
+  if (e == hipSuccess) {
+    // Every stage clears the bookkeeping of the stage before it once that
+    // stage is done, so this is the only initialisation ever needed.
+    chk(hipMemset(r->cull_status, 0, V * r->cull_tiles * 4));
+    chk(hipMemset(r->sort_status, 0, V * r->sort_words * kMaxDepthPasses * 4));
+    chk(hipMemset(r->bin_status, 0, V * r->bin_words * 8));
+    chk(hipMemset(r->pair_status, 0, V * r->pair_words * kMaxPairPasses * 4));
+    chk(hipMemset(r->counters, 0, V * kCounterWords * 4));
+    chk(hipMemset(r->depth_hist, 0, V * kMaxDepthPasses * 1024 * 4));
+    chk(hipMemset(r->pair_hist, 0, V * kMaxPairPasses * 256 * 4));
+    chk(hipDeviceSynchronize());
+  }
+  if (e != hipSuccess) {
+    renderer_free(r);
+    delete r;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+      return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: device memory for max_splat_count x max_views");
+    return fail(SPL_ERROR_DEVICE, std::string("spl_renderer_create: ") + hipGetErrorString(e));
+  }
+  *out = r;
+  return SPL_OK;
+}
+
+int spl_renderer_destroy(spl_renderer* r) {
+  if (!r) return SPL_OK;
+  (void)hipSetDevice(r->dev->ordinal);
+  (void)hipStreamSynchronize(r->dev->stream);
+  renderer_free(r);
+  delete r;
+  return SPL_OK;
+}
+
+int spl_renderer_get_config(const spl_renderer* r, spl_configuration* out) {
+  if (!r || !out) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_get_config: null argument");
+  *out = r->cfg;
+  return SPL_OK;
+}
+
+int spl_render_views(spl_renderer* r, spl_scene* s, uint32_t view_count, spl_frame* const* frames,
+                     const float* cams, const float* views) {
+  if (!r || !s || !frames || !cams || !views) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_render_views: null argument");
+  if (s->dev != r->dev) return fail(SPL_ERROR_INVALID_ARGUMENT, "scene and renderer must share a device");
+  if (s->count > r->cap) return fail(SPL_ERROR_CAPACITY, "scene has more splats than max_splat_count");
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  for (uint32_t first = 0; first < view_count; first += r->max_views) {
+    const uint32_t nv = std::min<uint32_t>(r->max_views, view_count - first);
+    int rc = enqueue_batch(r, s, nv, frames + first, cams + 16 * (size_t)first, views + 16 * (size_t)first);
+    if (rc) return rc;
+  }
+  return SPL_OK;
+}
+
+int spl_render_frame(spl_renderer* r, spl_scene* s, spl_frame* f, const float cam[16], const float view[16]) {
+  if (!r || !s || !f || !cam || !view) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_render_frame: null argument");
+  spl_frame* frames[1] = {f};
+  return spl_render_views(r, s, 1, frames, cam, view);
+}
+
+// ---------------------------------------------------------------------------
+// Parity and timing taps
+// ---------------------------------------------------------------------------
+
+int spl_debug_select_view(spl_renderer* r, uint32_t view) {
+  if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  if (view >= r->max_views || (r->have_frame && view >= r->last_views))
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "view outside the last batch");
+  r->debug_view = view;
+  return SPL_OK;
+}
+
+int spl_debug_visible_count(spl_renderer* r, uint64_t* out) {
+  if (!r || !out) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  uint32_t v = 0;
+  int rc = read_counter(r, r->debug_view, kVisible, &v);
+  *out = v;
+  return rc;
+}
+
+int spl_debug_pair_count(spl_renderer* r, uint64_t* out) {
+  if (!r || !out) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  uint32_t v = 0, o = 0;
+  int rc = read_counter(r, r->debug_view, kPairs, &v);
+  if (rc) return rc;
+  rc = read_counter(r, r->debug_view, kPairOverflow, &o);
+  *out = v;
+  if (rc) return rc;
+  if (o) return fail(SPL_ERROR_CAPACITY, "screen-tile pair capacity exceeded in the last frame");
+  return SPL_OK;
+}
+
+int spl_debug_read_entries(spl_renderer* r, uint32_t* keys, uint32_t* indices, uint64_t capacity, uint64_t* count) {
+  if (!r || !count) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  if (!r->have_frame) return fail(SPL_ERROR_INVALID_ARGUMENT, "no frame rendered yet");
+  std::vector<uint32_t> order, k;
+  int rc = draw_order(r, order, &k);
+  if (rc) return rc;
+  *count = order.size();
+  std::vector<uint32_t> idx(order.size());
+  if (!order.empty())
+    HIP_TRY(hipMemcpy(idx.data(), r->compact_idx + (size_t)r->debug_view * r->cap, idx.size() * 4,
+                      hipMemcpyDeviceToHost));
+  const size_t m = (size_t)std::min<uint64_t>(capacity, order.size());
+  for (size_t i = 0; i < m; ++i) {
+    if (keys) keys[i] = k[i];
+    if (indices) indices[i] = idx[order[i]];
+  }
+  return SPL_OK;
+}
+
+int spl_debug_read_records(spl_renderer* r, float* out, uint64_t capacity, uint64_t* count) {
+  if (!r || !count) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  if (!r->have_frame) return fail(SPL_ERROR_INVALID_ARGUMENT, "no frame rendered yet");
+  std::vector<uint32_t> order;
+  int rc = draw_order(r, order, nullptr);
+  if (rc) return rc;
+  *count = order.size();
+  std::vector<float4> rec(order.size() * kRecordFloat4);
+  if (!order.empty())
+    HIP_TRY(hipMemcpy(rec.data(), r->records + (size_t)r->debug_view * r->cap * kRecordFloat4, rec.size() * 16,
+                      hipMemcpyDeviceToHost));
+  const size_t m = (size_t)std::min<uint64_t>(capacity, order.size());
+  if (out)
+    for (size_t i = 0; i < m; ++i) {
+      const float4* p = &rec[(size_t)order[i] * kRecordFloat4];
+      const float v[10] = {p[0].x, p[0].y, p[0].z, p[0].w, p[1].x, p[1].y, p[1].z, p[1].w, p[2].x, p[2].y};
+      std::memcpy(out + 10 * i, v, sizeof(v));
+    }
+  return SPL_OK;
+}
+
+int spl_debug_enable_conics(spl_renderer* r, int enable) {
+  if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  if (enable && !r->conics) HIP_TRY(dev_alloc(&r->conics, r->cap * 6));
+  r->conics_on = enable != 0;
+  return SPL_OK;
+}
+
+int spl_debug_read_conics(spl_renderer* r, double* out, uint64_t capacity, uint64_t* count) {
+  if (!r || !count) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  if (!r->conics_on || !r->have_frame || r->debug_view != 0)
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "conic capture covers view 0 once enabled");
+  std::vector<uint32_t> order;
+  int rc = draw_order(r, order, nullptr);
+  if (rc) return rc;
+  *count = order.size();
+  std::vector<double> cn(order.size() * 6);
+  if (!order.empty()) HIP_TRY(hipMemcpy(cn.data(), r->conics, cn.size() * 8, hipMemcpyDeviceToHost));
+  const size_t m = (size_t)std::min<uint64_t>(capacity, order.size());
+  if (out)
+    for (size_t i = 0; i < m; ++i) std::memcpy(out + 6 * i, &cn[(size_t)order[i] * 6], 48);
+  return SPL_OK;
+}
+
+int spl_debug_enable_tile_stats(spl_renderer* r, int enable) {
+  if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  r->tile_stats_on = enable != 0;
+  return SPL_OK;
+}
+
+int spl_debug_read_tile_stats(spl_renderer* r, uint32_t* out, uint64_t capacity, uint64_t* count) {
+  if (!r || !count) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  if (!r->tile_stats_on || !r->tile_work || !r->have_frame)
+    return fail(SPL_ERROR_INVALID_ARGUMENT, "tile statistics not enabled");
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  *count = r->last_tiles[0];
+  const size_t m = (size_t)std::min<uint64_t>(capacity, r->last_tiles[0]);
+  if (out && m) HIP_TRY(hipMemcpy(out, r->tile_work, m * sizeof(uint2), hipMemcpyDeviceToHost));
+  return SPL_OK;
+}
+
+int spl_debug_enable_timing(spl_renderer* r, int enable) {
+  if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  r->timing = enable != 0;
+  r->ev_used = 0;
+  for (double& v : r->stage_sum_ms) v = 0.0;
+  r->stage_frames = 0;
+  return SPL_OK;
+}
+
+int spl_debug_stage_times(spl_renderer* r, float* ms, uint32_t capacity) {
+  if (!r || !ms) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  for (size_t f = 0; f < r->ev_used; ++f) {
+    for (int i = 0; i < kStageCount; ++i) {
+      float t = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&t, r->ev_pool[f][i], r->ev_pool[f][i + 1]));
+      r->stage_sum_ms[i] += t;
+    }
+    r->stage_frames += r->ev_views[f];
+  }
+  r->ev_used = 0;
+  for (uint32_t i = 0; i < capacity && i < (uint32_t)kStageCount; ++i)
+    ms[i] = r->stage_frames ? (float)(r->stage_sum_ms[i] / (double)r->stage_frames) : 0.0f;
+  return SPL_OK;
+}
+
+}  // extern "C"

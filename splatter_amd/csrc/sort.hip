@@ -11,9 +11,10 @@
 //  - tiles are TILE = THREADS*ITEMS keys (4096) taken in order from an
 //    atomic counter by a persistent grid (reference: 1024-key tiles, one
 //    workgroup each, shaders.wgsl:406-420).
-//  - the global digit offsets come from the histogram gathered by the
-//    producer kernel and are scanned by every block on entry (radixSortB's
-//    serial scan disappears).
+//  - the global digit offsets come from the upsweep histogram and are
+//    scanned by every block on entry (radixSortB's serial scan disappears).
+//  - the views of a batch sort side by side (view = blockIdx.y), each with
+//    its own tile counter and status words.
 //  - decoupled look-back per digit with agent-scope status words
 //    (aggregate / inclusive flags + 30-bit count, as shaders.wgsl:438-457).
 //  - keys, then values, are reordered through LDS so that global stores of
@@ -40,10 +41,19 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & (kWave - 1), w = tid / kWave;
-  clear_words(a.clear_a, a.clear_a_words);
-  clear_words(a.clear_b, a.clear_b_words);
+  const uint32_t v = blockIdx.y;
+  if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
+  if (a.clear_b) clear_words(a.clear_b + v * a.clear_b_stride, a.clear_b_words);
+  uint32_t* counters = a.counters + (size_t)v * kCounterWords;
+  const uint32_t* keys_in = a.keys_in + v * a.entry_stride;
+  const uint32_t* vals_in = a.vals_in ? a.vals_in + v * a.entry_stride : nullptr;
+  uint32_t* keys_out = a.keys_out + v * a.entry_stride;
+  uint32_t* vals_out = a.vals_out + v * a.entry_stride;
+  const uint32_t* hist = a.hist + v * a.hist_stride;
+  uint32_t* status = a.status + v * a.status_stride;
+  uint32_t* tile_counter = counters + a.tile_slot;
 
-  uint32_t n = *a.count;
+  uint32_t n = counters[a.count_slot];
   if (n > a.count_cap) n = a.count_cap;
   const uint32_t num_tiles = (n + TILE - 1) / TILE;
   const int shift = a.shift;
@@ -53,7 +63,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
       const uint32_t d = tid * DPT + k;
-      v[k] = d < (uint32_t)RADIX ? a.hist[d] : 0u;
+      v[k] = d < (uint32_t)RADIX ? hist[d] : 0u;
       sum += v[k];
     }
     uint32_t total;
@@ -67,7 +77,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
   }
 
   while (true) {
-    if (tid == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     for (int i = tid; i < WAVES * RADIX; i += THREADS) s_wave_hist[i] = 0u;
     __syncthreads();
     const uint32_t t = s_tile;
@@ -81,8 +91,8 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t idx = wave_base + j * kWave;
       const bool valid = idx < n;
-      key[j] = valid ? a.keys_in[idx] : 0xFFFFFFFFu;
-      val[j] = valid ? (a.vals_in ? a.vals_in[idx] : idx) : 0u;
+      key[j] = valid ? keys_in[idx] : 0xFFFFFFFFu;
+      val[j] = valid ? (vals_in ? vals_in[idx] : idx) : 0u;
     }
 
     // WLMS ranking, items in input order
@@ -131,7 +141,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
 #pragma unroll
       for (int k = 0; k < DPT; ++k) {
         const uint32_t d = tid * DPT + k;
-        if (d < (uint32_t)RADIX) store_status(a.status + (size_t)t * RADIX + d, kFlagAggregate | cnt_d[k]);
+        if (d < (uint32_t)RADIX) store_status(status + (size_t)t * RADIX + d, kFlagAggregate | cnt_d[k]);
       }
     }
     uint32_t total;
@@ -140,8 +150,8 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
     for (int k = 0; k < DPT; ++k) {
       const uint32_t d = tid * DPT + k;
       if (d < (uint32_t)RADIX) {
-        const uint32_t prefix = t == 0 ? 0u : lookback_window<32>(a.status + d, t, RADIX);
-        store_status(a.status + (size_t)t * RADIX + d, kFlagInclusive | (prefix + cnt_d[k]));
+        const uint32_t prefix = t == 0 ? 0u : lookback_window<32>(status + d, t, RADIX);
+        store_status(status + (size_t)t * RADIX + d, kFlagInclusive | (prefix + cnt_d[k]));
         s_local[d] = excl;
         s_base[d] = s_digit_start[d] + prefix - excl;
       }
@@ -168,8 +178,8 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
         out[k] = s_base[(kk >> shift) & MASK] + i;
         // a position past n means the histogram and the keys disagree:
         // report it instead of storing out of bounds
-        if (out[k] < n) a.keys_out[out[k]] = kk;
-        else *a.error_flag = 1u;
+        if (out[k] < n) keys_out[out[k]] = kk;
+        else counters[kScatterError] = 1u;
       }
     }
     __syncthreads();
@@ -180,18 +190,18 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const uint32_t i = k * THREADS + tid;
-      if (i < tile_n && out[k] < n) a.vals_out[out[k]] = s_buf[i];
+      if (i < tile_n && out[k] < n) vals_out[out[k]] = s_buf[i];
     }
     __syncthreads();
   }
-  finish_persistent(a.tile_counter, a.done_counter);
+  finish_persistent(tile_counter, counters + kDone + a.tile_slot);
 }
 
 void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s) {
   if (bits == 10)
-    hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreads, kSortItems>), dim3(a.grid), dim3(kSortThreads), 0, s, a);
+    hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreads, kSortItems>), dim3(a.grid_x, a.views), dim3(kSortThreads), 0, s, a);
   else
-    hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreads, kSortItems>), dim3(a.grid), dim3(kSortThreads), 0, s, a);
+    hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreads, kSortItems>), dim3(a.grid_x, a.views), dim3(kSortThreads), 0, s, a);
 }
 
 }  // namespace spl

@@ -1,5 +1,10 @@
 // spl_internal.h -- host-side declarations shared by the C-ABI layer and the
 // kernel translation units.
+//
+// Every stage runs for a batch of views at once: per-view buffers live in
+// one allocation each, view v at base + v * stride, and kernels take the view
+// from blockIdx.y (or loop over views where one read of the scene serves all
+// of them: cull and project).  A single frame is the batch of one.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
@@ -24,11 +29,12 @@ struct Uniforms {
 constexpr int kTileSize = 16;        // raster tile edge in pixels (16x16 = 256 lanes)
 constexpr int kMaxDepthPasses = 4;   // 30-bit depth keys: 4 x 8 bits or 3 x 10 bits
 constexpr int kMaxPairPasses = 3;    // screen-tile ids up to 24 bits
-constexpr int kRecordFloat4 = 3;     // 48-byte record: rgba | A00 A10 A01 A11 | tx ty rect rect
+constexpr int kRecordFloat4 = 3;     // 48-byte record: rgba | A00 A10 A01 A11 | tx ty box box
+constexpr int kMaxViews = 32;        // views per launch (one bit each in the visibility masks)
 
-// Counter block layout (u32 words).
+// Counter block layout (u32 words), one block per view.
 enum CounterSlot : int {
-  kCtrCull = 0,
+  kCtrCull = 0,     // view 0's block holds the shared multi-view cull counters
   kCtrSort0 = 1,    // .. kCtrSort0 + 3
   kCtrBin = 5,
   kCtrPair0 = 6,    // .. kCtrPair0 + 2
@@ -50,101 +56,115 @@ struct SceneView {
   uint32_t count;
 };
 
+constexpr int kCullThreads = 256, kCullItems = 16;   // 4096 splats per cull / project tile
+constexpr int kSortThreads = 256, kSortItems = 16;   // 4096 keys per sort tile
+constexpr int kBinThreads = 256, kBinItems = 4;      // 1024 draw ranks per bin tile
+constexpr int kCullTile = kCullThreads * kCullItems;
+
 struct CullLaunch {
   const float* centers;
   uint32_t n;
-  float vp[16];
-  float tol;
-  int gpu_predicate;        // 1: shaders.wgsl:54-61 (DepthSorting::None), 0: renderer.rs:381-387
-  uint32_t* keys_out;
-  uint32_t* idx_out;
-  uint32_t* hist;           // [passes][1 << bits], accumulated
-  int passes, bits;
-  uint32_t* status;
-  uint32_t* counters;
+  uint32_t views;
+  const Uniforms* uniforms;  // [views]
+  int gpu_predicate;         // 1: shaders.wgsl:54-61 (DepthSorting::None), 0: renderer.rs:381-387
+  int write_keys;
+  uint32_t* keys; size_t keys_stride;        // view v: keys + v * keys_stride
+  uint32_t* idx; size_t idx_stride;
+  uint32_t* vis;                             // [n] visibility mask over views
+  uint32_t* tile_prefix; size_t tp_stride;   // [views][cull tiles] exclusive prefix
+  uint32_t* status; size_t status_stride;    // [views][cull tiles]
+  uint32_t* counters;                        // [views][kCounterWords]
   int grid;
+};
+
+struct HistLaunch {
+  const uint32_t* keys; size_t keys_stride;
+  uint32_t* hist; size_t hist_stride;        // [views][passes << bits]
+  uint32_t* counters;
+  int passes, bits;
+  uint32_t* clear; size_t clear_words; size_t clear_stride;  // per view
+  int grid_x, views;
 };
 
 struct SortPassLaunch {
   const uint32_t* keys_in;
-  const uint32_t* vals_in;  // null: values are the input positions
+  const uint32_t* vals_in;   // null: values are the input positions
   uint32_t* keys_out;
   uint32_t* vals_out;
-  const uint32_t* count;    // device count of entries
+  size_t entry_stride;       // per view, all four arrays
+  uint32_t* counters;        // [views][kCounterWords]
+  int count_slot;            // kVisible or kPairs
   uint32_t count_cap;
   int shift;
-  const uint32_t* hist;     // this pass's global histogram
-  uint32_t* status;         // [tiles][radix]
-  uint32_t* tile_counter;
-  uint32_t* done_counter;
-  uint32_t* clear_a; size_t clear_a_words;
-  uint32_t* clear_b; size_t clear_b_words;
-  uint32_t* error_flag;     // set when a scatter position falls outside [0, count)
-  int grid;
+  const uint32_t* hist; size_t hist_stride;  // this pass's histogram, per view
+  uint32_t* status; size_t status_stride;    // [tiles][radix], per view
+  int tile_slot;             // counter slot of this pass's tile counter
+  uint32_t* clear_a; size_t clear_a_words; size_t clear_a_stride;
+  uint32_t* clear_b; size_t clear_b_words; size_t clear_b_stride;
+  int grid_x, views;
 };
 
 struct ProjectLaunch {
   SceneView scene;
-  const uint32_t* compact_idx;
-  const uint32_t* count;
-  Uniforms u;
+  uint32_t views;
+  const Uniforms* uniforms;
+  const uint32_t* vis;
+  const uint32_t* tile_prefix; size_t tp_stride;
   int sh_order;
   int use_covariance_for_scale;
   int use_unaligned_rectangles;
-  float4* records;
-  double* conics;           // optional, 6 per entry
-  uint32_t* clear_a; size_t clear_a_words;
-  uint32_t* clear_b; size_t clear_b_words;
+  float4* records; size_t rec_stride;       // float4 units per view
+  double* conics;                           // optional, view 0 only, 6 per entry
+  uint32_t* clear_a; size_t clear_a_words; size_t clear_a_stride;
+  uint32_t* clear_b; size_t clear_b_words; size_t clear_b_stride;
   int grid;
 };
 
 struct BinLaunch {
-  const uint32_t* order;    // record index per draw rank; null = identity
-  const float4* records;
-  const uint32_t* count;    // visible count
-  uint32_t tiles_x;
+  const uint32_t* order; size_t order_stride;  // record index per draw rank; null = identity
+  const float4* records; size_t rec_stride;
+  const Uniforms* uniforms;
   uint32_t* pair_keys;
-  uint32_t* pair_vals;
+  uint32_t* pair_vals; size_t pair_stride;
   uint32_t pair_cap;
-  uint32_t* hist;           // [pair passes][256], accumulated
+  uint32_t* hist; size_t hist_stride;          // [pair passes][256] per view, accumulated
   int pair_passes;
-  uint64_t* status;
+  uint64_t* status; size_t status_stride;
   uint32_t* counters;
-  int grid;
+  int grid_x, views;
 };
 
 struct RangesLaunch {
-  const uint32_t* pair_keys;
-  const uint32_t* counters;
+  const uint32_t* pair_keys; size_t pair_stride;
+  uint32_t* counters;
   uint32_t pair_cap;
-  uint32_t tiles;
-  uint2* ranges;
-  uint32_t* clear_a; size_t clear_a_words;
-  uint32_t* clear_b; size_t clear_b_words;
+  const Uniforms* uniforms;
+  uint2* ranges; size_t ranges_stride;
+  uint32_t* clear_a; size_t clear_a_words; size_t clear_a_stride;
+  uint32_t* clear_b; size_t clear_b_words; size_t clear_b_stride;
+  int grid_x, views;
 };
 
 struct RasterLaunch {
-  const uint32_t* pair_vals;
-  const uint32_t* visible;  // record count: pair values at or past it are ignored
-  const float4* records;
-  uint2* ranges;
-  Uniforms u;
+  const uint32_t* pair_vals; size_t pair_stride;
+  const float4* records; size_t rec_stride;
+  uint2* ranges; size_t ranges_stride;
+  const Uniforms* uniforms;
+  uint32_t* const* pixels;   // [views] frame buffers
+  uint32_t* counters;
   int use_unaligned_rectangles;
   int blend_f32;
   int bgra;
-  uint32_t* pixels;
-  uint2* tile_work;         // optional: per tile (list length, entries consumed)
+  uint2* tile_work;          // optional, view 0: per tile (list length, entries consumed)
+  int grid_x, views;
 };
 
-constexpr int kCullThreads = 256, kCullItems = 16;
-constexpr int kSortThreads = 256, kSortItems = 16;
-constexpr int kBinThreads = 256, kBinItems = 4;
-
 void launch_cull(const CullLaunch& a, hipStream_t s);
+void launch_histograms(const HistLaunch& a, hipStream_t s);
 void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s);
 void launch_project(const ProjectLaunch& a, hipStream_t s);
 void launch_bin(const BinLaunch& a, hipStream_t s);
-void launch_ranges(const RangesLaunch& a, uint32_t max_pairs, hipStream_t s);
+void launch_ranges(const RangesLaunch& a, hipStream_t s);
 void launch_raster(const RasterLaunch& a, hipStream_t s);
 
 }  // namespace spl

@@ -1,0 +1,62 @@
+// spl_objects.h -- the objects behind the C ABI's opaque handles and the
+// error plumbing shared by api.cpp and renderer.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "splatter.h"
+
+namespace spl {
+
+// Sets the calling thread's message (spl_last_error) and returns `code`.
+int fail(int code, const std::string& msg);
+
+// Renderer accessors for the scene functions (Scene::new clamps to the
+// renderer's max_splat_count, scene.rs:46).
+spl_device* renderer_device(const spl_renderer* r);
+uint64_t renderer_capacity(const spl_renderer* r);
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+template <typename T>
+hipError_t dev_alloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, count * sizeof(T));
+}
+
+template <typename T>
+void dev_free(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+}  // namespace spl
+
+#define HIP_TRY(expr)                                                                                   \
+  do {                                                                                                  \
+    hipError_t e_ = (expr);                                                                             \
+    if (e_ != hipSuccess) return spl::fail(SPL_ERROR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct spl_device {
+  int ordinal = 0;
+  hipStream_t stream = nullptr;
+  int cu_count = 0;
+};
+
+struct spl_scene {
+  spl_device* dev = nullptr;
+  uint64_t count = 0;
+  float* centers = nullptr;
+  float4* rotations = nullptr;
+  float4* scale_alpha = nullptr;
+  float* sh = nullptr;
+};
+
+struct spl_frame {
+  spl_device* dev = nullptr;
+  uint32_t width = 0, height = 0;
+  uint32_t* pixels = nullptr;
+};

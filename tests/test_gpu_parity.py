@@ -148,13 +148,32 @@ def test_deterministic_and_frame_size_changes():
     assert np.array_equal(a, b) and np.array_equal(ka, kb) and np.array_equal(ia, ib)
 
 
-def test_render_views_equals_single_frames():
+@pytest.mark.parametrize("graphs", [True, False])
+def test_render_views_equals_single_frames(graphs):
+    """Batched form: uneven frame sizes in one launch sequence, more views
+    than max_views (two batches), each view equal to its single render and
+    to the oracle (entries bit-exact, image within tolerance)."""
     w = Workload("clustered", 300_000, seed=8)
-    case = Case(w, config(), 640, 360, orbit_poses(4)[0])
-    poses = orbit_poses(4)
-    frames = [sa.Frame(device(), 640, 360) for _ in poses]
-    case.renderer.render_views(frames, [p[0] for p in poses], [p[1] for p in poses], case.scene)
+    case = Case(w, config(max_views=4, use_hip_graphs=graphs), 640, 360, orbit_poses(6)[0])
+    poses = orbit_poses(6)
+    sizes = [(640, 360), (320, 240), (1280, 720), (97, 61), (640, 360), (500, 500)]
+    frames = [sa.Frame(device(), wd, ht) for wd, ht in sizes]
+    for rep in range(2):  # second replay of the same shapes
+        case.renderer.render_views(frames, [p[0] for p in poses], [p[1] for p in poses], case.scene)
     batched = [f.read() for f in frames]
-    for (cam, view), img in zip(poses, batched):
-        case.renderer.render_frame(case.frame, cam, view, case.scene)
-        assert np.array_equal(case.frame.read(), img)
+    # entries of the views of the second batch (views 4, 5 -> slots 0, 1)
+    for slot, vi in ((0, 4), (1, 5)):
+        case.renderer.select_view(slot)
+        keys, idx = case.renderer.read_entries()
+        cam, view = poses[vi]
+        u = ob.uniforms(cam, view, *sizes[vi])
+        okeys, oidx = ob.cpu_depth_sort(case.splats["center"], ob.vp_of(u), 1.1)
+        assert np.array_equal(idx, oidx) and np.array_equal(keys, okeys)
+    case.renderer.select_view(0)
+    for (cam, view), img, f in zip(poses, batched, frames):
+        case.renderer.render_frame(f, cam, view, case.scene)
+        assert np.array_equal(f.read(), img)
+    cam, view = poses[2]
+    u = ob.uniforms(cam, view, *sizes[2])
+    _, _, _, oimg = ob.render_frame(case.splats, u, 3, True)
+    assert image_diff(batched[2][..., [2, 1, 0, 3]], oimg) <= IMAGE_TOL
