@@ -273,13 +273,13 @@ void orc_contour_f64(const double scale[3], const double rotation[4], const doub
 /* fragment (shaders.wgsl:580-592) + fixed-function raster and blend       */
 /* ---------------------------------------------------------------------- */
 
-/* exp(x) by Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, and the
- * degree-7 Taylor polynomial of e^r in Horner form, scaled by 2^k through
- * the exponent field.  Only IEEE add/mul/floor, so it rounds identically on
- * any IEEE-754 machine without contraction. */
+/* e^max(x, -86) for x <= 0 (the fragment's -power/2) by Cody-Waite
+ * reduction x = k ln2 + r, |r| <= ln2/2, and the degree-7 Taylor polynomial
+ * of e^r in Horner form, scaled by 2^k through the exponent field.  Only IEEE
+ * add/mul/floor/max, so it rounds identically on any IEEE-754 machine without
+ * contraction.  Below -86 every alpha is far under the 1/255 threshold. */
 float orc_exp(float x) {
-    if (!(x > -87.0f)) return 0.0f;
-    if (x > 88.0f) return INFINITY;
+    x = fmaxf(x, -86.0f);
     float kf = floorf(x * 1.44269502f + 0.5f);
     float r = (x - kf * 0.693145752f) - kf * 1.42860677e-06f;
     float p = 0.000198412701f;

@@ -3,7 +3,7 @@
 //
 // This is the per-splat half of the reference's vertex stage
 // (shaders.wgsl:507-547) evaluated once per visible splat instead of once per
-// quad vertex.  Entry c of the compacted list (splat compact_idx[c]) writes
+// quad vertex.  Entry c of a view's compacted list writes that view's
 // record c: rgba | A00 A10 A01 A11 | tx ty bbox, 48 B, where A = (col0 col1)
 // and t are the rectangle's affine map (shaders.wgsl:543-547) in view-plane
 // units and bbox is a conservative pixel box of what the rectangle can cover.
@@ -214,16 +214,21 @@ __device__ void pixel_box(float A00, float A10, float A01, float A11, float tx, 
 
 }  // namespace
 
-// One record per (visible splat, view): the splat's attributes are read
-// once and projected into every view of the batch whose cull kept it.  Entry
-// c of view v's list (its c-th visible splat by index) is found from the
-// visibility masks and the cull's per-tile prefixes, so records land where
-// the cull put the keys, with no per-view index to read.
+// One record per (visible splat, view).  Entry c of view v's list (its c-th
+// visible splat by index) is found from the visibility masks and the cull's
+// per-tile prefixes, so records land where the cull put the keys, with no
+// per-view index to read.  For each group of 256 splats of a tile the block
+// lists its (splat, view) pairs with a scan and every lane projects one pair,
+// so lanes do not idle on views that do not see their splat; the lanes of
+// one splat read its attributes from the same cache lines.
 template <int L>
 __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
   constexpr int THREADS = kCullThreads, ITEMS = kCullItems, WAVES = THREADS / kWave, TILE = kCullTile;
   __shared__ uint32_t s_off[kMaxViews][ITEMS * WAVES];
   __shared__ uint32_t s_tp[kMaxViews];
+  __shared__ uint64_t s_bal[kMaxViews][WAVES];
+  __shared__ uint16_t s_items[THREADS * kMaxViews];
+  __shared__ uint32_t s_tmp[WAVES];
   const uint32_t nv = a.views;
   for (uint32_t v = 0; v < nv; ++v) {
     if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
@@ -234,6 +239,7 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
   const uint32_t num_tiles = (n + TILE - 1) / TILE;
   constexpr int SH_FLOATS = 3 * (L + 1) * (L + 1);
   constexpr int SH_F4 = (SH_FLOATS + 3) / 4;
+  const double splat_scale = (double)a.uniforms[0].splat_scale;
 
   for (uint32_t t = blockIdx.x; t < num_tiles; t += gridDim.x) {
     // per view: exclusive offsets of the (item, wave) groups inside the tile
@@ -261,42 +267,37 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
     __syncthreads();
 
     for (int j = 0; j < ITEMS; ++j) {
-      const uint32_t i = t * TILE + j * THREADS + tid;
-      const uint32_t m = i < n ? a.vis[i] : 0u;
-      if (__ballot(m != 0u) == 0ull) continue;
-      float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-      float4 rq = make_float4(0, 0, 0, 0), sa = rq;
-      float sh[SH_F4 * 4];
-#pragma unroll
-      for (int k = 0; k < SH_F4 * 4; ++k) sh[k] = 0.0f;
-      if (m) {
-        cx = a.scene.centers[3 * (size_t)i + 0];
-        cy = a.scene.centers[3 * (size_t)i + 1];
-        cz = a.scene.centers[3 * (size_t)i + 2];
-        rq = a.scene.rotations[i];
-        sa = a.scene.scale_alpha[i];
+      const uint32_t base = t * TILE + j * THREADS;
+      const uint32_t mine = base + tid < n ? a.vis[base + tid] : 0u;
+      for (uint32_t v = 0; v < nv; ++v) {
+        const uint64_t b = __ballot((mine >> v) & 1u);
+        if (lane == 0) s_bal[v][w] = b;
+      }
+      uint32_t total;
+      uint32_t slot = block_exclusive_scan<THREADS>((uint32_t)__popc(mine), s_tmp, total);
+      for (uint32_t m = mine; m; m &= m - 1) s_items[slot++] = (uint16_t)((tid << 5) | (uint32_t)__ffs(m) - 1u);
+      __syncthreads();
+      for (uint32_t k = tid; k < total; k += THREADS) {
+        const uint32_t it = s_items[k];
+        const uint32_t sl = it >> 5, v = it & 31u;
+        const uint32_t i = base + sl;
+        const uint32_t sw = sl / kWave, sbit = sl % kWave;
+        const uint32_t c = s_tp[v] + s_off[v][j * WAVES + sw] +
+                           (uint32_t)__popcll(s_bal[v][sw] & ((1ull << sbit) - 1ull));
+        const Uniforms& u = a.uniforms[v];
+
+        const float cx = a.scene.centers[3 * (size_t)i + 0];
+        const float cy = a.scene.centers[3 * (size_t)i + 1];
+        const float cz = a.scene.centers[3 * (size_t)i + 2];
+        const float4 rq = a.scene.rotations[i];
+        const float4 sa = a.scene.scale_alpha[i];
+        float sh[SH_F4 * 4];
         const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
 #pragma unroll
-        for (int k = 0; k < SH_F4; ++k) {
-          const float4 q = shp[k];
-          sh[4 * k + 0] = q.x; sh[4 * k + 1] = q.y; sh[4 * k + 2] = q.z; sh[4 * k + 3] = q.w;
+        for (int q = 0; q < SH_F4; ++q) {
+          const float4 x = shp[q];
+          sh[4 * q + 0] = x.x; sh[4 * q + 1] = x.y; sh[4 * q + 2] = x.z; sh[4 * q + 3] = x.w;
         }
-      }
-      // view-independent parts (shaders.wgsl:63-79,101-104,127-130)
-      const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
-      const double pos[3] = {(double)cx, (double)cy, (double)cz};
-      const double splat_scale = (double)a.uniforms[0].splat_scale;
-      const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
-      double Tc[3][3], Tv[3][3];
-      contour_transform(scale, rot, Tc);
-      if (a.use_covariance_for_scale) covariance_transform(scale, rot, Tv);
-
-      for (uint32_t v = 0; v < nv; ++v) {
-        const bool mine = (m >> v) & 1u;
-        const uint64_t bal = __ballot(mine);
-        if (!mine) continue;
-        const uint32_t c = s_tp[v] + s_off[v][j * WAVES + w] + lanes_below(bal);
-        const Uniforms& u = a.uniforms[v];
 
         // shaders.wgsl:531-532: colour from the normalised ray direction
         float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
@@ -306,8 +307,15 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
         sh_color<L>(rd, sh, rgb);
 
         // shaders.wgsl:533-547
+        const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
+        const double pos[3] = {(double)cx, (double)cy, (double)cz};
+        const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
         double M[3][3];
-        contour(Tc, pos, u, M);
+        {
+          double Tc[3][3];
+          contour_transform(scale, rot, Tc);
+          contour(Tc, pos, u, M);
+        }
         // extractTranslationOfEllipse (shaders.wgsl:179-191)
         const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
         const double inv_disc = 1.0 / disc;
@@ -325,12 +333,13 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
         double s0, s1;
         if (a.use_covariance_for_scale) {
           // extractScaleOfCovariance (shaders.wgsl:240-246)
-          double cov[3][3];
+          double Tv[3][3], cov[3][3];
+          covariance_transform(scale, rot, Tv);
           covariance(Tv, pos, u, cov);
           const double cd = cov[0][0] - cov[1][1];
-          const double b = sqrt(cd * cd + (4.0 * cov[0][1]) * cov[0][1]);
-          s0 = sqrt(((cov[0][0] + cov[1][1]) + b) * 0.5);
-          s1 = sqrt(((cov[0][0] + cov[1][1]) - b) * 0.5);
+          const double bb = sqrt(cd * cd + (4.0 * cov[0][1]) * cov[0][1]);
+          s0 = sqrt(((cov[0][0] + cov[1][1]) + bb) * 0.5);
+          s1 = sqrt(((cov[0][0] + cov[1][1]) - bb) * 0.5);
         } else {
           // extractScaleOfEllipse (shaders.wgsl:217-237)
           const double d = ((2.0 * M[0][1]) * jj) * kk;
@@ -355,8 +364,8 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
           cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 

@@ -20,12 +20,13 @@ namespace spl {
 
 namespace {
 
-// exp(x): Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, degree-7 Taylor
-// polynomial of e^r in Horner form, 2^k through the exponent field.  Only
-// IEEE add/mul/floor, so it rounds the same on the CPU and the GPU.
+// e^max(x, -86) for x <= 0 (the fragment's -power/2): Cody-Waite reduction
+// x = k ln2 + r, |r| <= ln2/2, degree-7 Taylor polynomial of e^r in Horner
+// form, 2^k through the exponent field.  Only IEEE add/mul/floor/max, so it
+// rounds the same on the CPU and the GPU.  Below -86 every alpha is far under
+// the 1/255 fragment threshold, so the clamp changes no pixel.
 __device__ __forceinline__ float spl_exp(float x) {
-  if (!(x > -87.0f)) return 0.0f;
-  if (x > 88.0f) return __int_as_float(0x7F800000);
+  x = fmaxf(x, -86.0f);
   const float kf = floorf(x * 1.44269502f + 0.5f);
   const float r = (x - kf * 0.693145752f) - kf * 1.42860677e-06f;
   float p = 0.000198412701f;
@@ -204,6 +205,13 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
 
   const float qx = ((((float)px + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
   const float qy = (1.0f - (((float)py + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
+  // view-plane extremes of the quadrant's pixel centres (x grows with px,
+  // y falls with py); only centres inside the image count
+  const uint32_t qx1 = min(qx0 + 7u, W - 1u), qy1 = min(qy0 + 7u, H - 1u);
+  const float2 q_lo = make_float2(((((float)qx0 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
+                                  (1.0f - (((float)qy1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
+  const float2 q_hi = make_float2(((((float)qx1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
+                                  (1.0f - (((float)qy0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
   const bool f32 = a.blend_f32 != 0;
   const bool unaligned = a.use_unaligned_rectangles != 0;
   const float margin = u.ellipse_margin;
@@ -246,7 +254,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     if (more) fetch(base + B);
     consumed += cnt;
     if (!wave_done) {
-      // entries of this batch whose pixel box meets the quadrant, in order
+      // entries of this batch that can cover a pixel centre of the quadrant,
+      // in order: pixel box against the quadrant, then the exact range of
+      // the coverage coordinate over the quadrant's pixel centres (it is
+      // affine in the centre, so its extremes sit at the corner centres)
       uint32_t n = 0;
 #pragma unroll
       for (int k = 0; k < B / 64; ++k) {
@@ -256,21 +267,38 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           const uint2 bx = s_box[buf][j];
           const uint32_t x0 = bx.x & 0xFFFFu, y0 = bx.x >> 16, x1 = bx.y & 0xFFFFu, y1 = bx.y >> 16;
           ov = bx.x != 0xFFFFFFFFu && x0 <= qx0 + 7u && x1 >= qx0 && y0 <= qy0 + 7u && y1 >= qy0;
+          if (ov) {
+            const float4 m = s_m[buf][j];
+            const float4 t = s_t[buf][j];
+            const float dxl = q_lo.x - t.x, dxh = q_hi.x - t.x, dyl = q_lo.y - t.y, dyh = q_hi.y - t.y;
+            float ax0, ax1, ay0, ay1;  // range of the tested coordinate pair
+            if (unaligned) {
+              ax0 = fminf(m.x * dxl, m.x * dxh) + fminf(m.y * dyl, m.y * dyh);
+              ax1 = fmaxf(m.x * dxl, m.x * dxh) + fmaxf(m.y * dyl, m.y * dyh);
+              ay0 = fminf(m.z * dxl, m.z * dxh) + fminf(m.w * dyl, m.w * dyh);
+              ay1 = fmaxf(m.z * dxl, m.z * dxh) + fmaxf(m.w * dyl, m.w * dyh);
+            } else {
+              ax0 = dxl; ax1 = dxh; ay0 = dyl; ay1 = dyh;
+            }
+            // slack far above the rounding of the per-pixel evaluation
+            const float h = t.z;
+            const float sx = 1e-4f * fmaxf(fmaxf(fabsf(ax0), fabsf(ax1)), h) + 1e-30f;
+            const float sy = 1e-4f * fmaxf(fmaxf(fabsf(ay0), fabsf(ay1)), h) + 1e-30f;
+            ov = ax1 + sx >= -h && ax0 - sx < h && ay1 + sy >= -h && ay0 - sy < h;
+          }
         }
-        const uint64_t m = __ballot(ov);
-        if (ov) s_list[w][n + lanes_below(m)] = (uint8_t)j;
-        n += (uint32_t)__popcll(m);
+        const uint64_t bm = __ballot(ov);
+        if (ov) s_list[w][n + lanes_below(bm)] = (uint8_t)j;
+        n += (uint32_t)__popcll(bm);
       }
       __builtin_amdgcn_wave_barrier();
       // four list slots per step: their LDS reads are issued together
       for (uint32_t i = 0; i < n; i += 4) {
         const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][i]);
-        uint2 bx[4];
         float4 m[4], t[4], col[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const uint32_t j = (js >> (8 * k)) & 255u;  // slots past n hold stale, in-range indices
-          bx[k] = s_box[buf][j];
           m[k] = s_m[buf][j];
           t[k] = s_t[buf][j];
           col[k] = s_c[buf][j];
@@ -278,15 +306,13 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           if (i + k >= n) break;
-          const bool inbox = px >= (bx[k].x & 0xFFFFu) && px <= (bx[k].y & 0xFFFFu) &&
-                             py >= (bx[k].x >> 16) && py <= (bx[k].y >> 16);
           const float dx = qx - t[k].x, dy = qy - t[k].y;
           const float tcx = m[k].x * dx + m[k].y * dy;
           const float tcy = m[k].z * dx + m[k].w * dy;
           const float h = t[k].z;
           const bool in = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
                                     : (dx >= -h && dx < h && dy >= -h && dy < h);
-          if (inbox && in && tr != 0.0f) {
+          if (in && tr != 0.0f) {
             const float power = tcx * tcx + tcy * tcy;
             const float alpha = t[k].w * spl_exp(-0.5f * power);
             if (alpha >= 1.0f / 255.0f) {
