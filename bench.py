@@ -88,6 +88,7 @@ def main() -> int:
     ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--radix-bits", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -102,7 +103,8 @@ def main() -> int:
     W, H, V_PER = args.width, args.height, args.views
     L = 3
     dev = sa.Device(0)
-    cfg = sa.Configuration(spherical_harmonics_order=L, max_splat_count=args.splats, max_views=V_PER)
+    cfg = sa.Configuration(spherical_harmonics_order=L, max_splat_count=args.splats, max_views=V_PER,
+                           radix_bits_per_digit=args.radix_bits)
     r = sa.Renderer(dev, cfg)
     wl = Workload("clustered", args.splats, seed=2)
     t0 = time.perf_counter()

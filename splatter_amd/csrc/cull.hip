@@ -145,26 +145,35 @@ __global__ __launch_bounds__(THREADS) void k_cull_views(CullLaunch a) {
 }
 
 // Onesweep upsweep: the digit histograms of every radix pass over a view's
-// keys, LDS first, then one global atomic per bin and block.
+// keys, LDS first, then one global atomic per bin and block.  Depth keys pile
+// into few top digits (their float exponents), so each bin is replicated
+// REP times across lanes to keep same-address LDS atomics apart.
 template <int BITS>
 __global__ __launch_bounds__(256) void k_digit_histograms(HistLaunch a) {
   constexpr int RADIX = 1 << BITS;
-  __shared__ uint32_t s_hist[kMaxDepthPasses * RADIX];
+  constexpr int REP = BITS <= 8 ? 8 : 2;
+  __shared__ uint32_t s_hist[kMaxDepthPasses * RADIX * REP];
   const uint32_t v = blockIdx.y;
   if (a.clear) clear_words(a.clear + v * a.clear_stride, a.clear_words);
   const uint32_t* keys = a.keys + v * a.keys_stride;
   uint32_t* hist = a.hist + v * a.hist_stride;
   const uint32_t n = a.counters[(size_t)v * kCounterWords + kVisible];
   const int words = a.passes * RADIX;
-  for (int i = threadIdx.x; i < words; i += blockDim.x) s_hist[i] = 0u;
+  for (int i = threadIdx.x; i < words * REP; i += blockDim.x) s_hist[i] = 0u;
   __syncthreads();
+  const uint32_t rep = threadIdx.x % REP;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const uint32_t k = keys[i];
-    for (int p = 0; p < a.passes; ++p) atomicAdd(&s_hist[p * RADIX + ((k >> (p * BITS)) & (RADIX - 1))], 1u);
+    for (int p = 0; p < a.passes; ++p)
+      atomicAdd(&s_hist[(p * RADIX + ((k >> (p * BITS)) & (RADIX - 1))) * REP + rep], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < words; i += blockDim.x)
-    if (s_hist[i]) atomicAdd(&hist[i], s_hist[i]);
+  for (int i = threadIdx.x; i < words; i += blockDim.x) {
+    uint32_t sum = 0;
+#pragma unroll
+    for (int r = 0; r < REP; ++r) sum += s_hist[i * REP + r];
+    if (sum) atomicAdd(&hist[i], sum);
+  }
 }
 
 void launch_cull(const CullLaunch& a, hipStream_t s) {

@@ -358,6 +358,7 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
         rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
         rec[1] = make_float4(A00, A10, A01, A11);
         rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
+        a.boxes[v * a.box_stride + c] = make_uint2(box_lo, box_hi);
         if (a.conics && v == 0) {
           double* cn = a.conics + 6 * (size_t)c;
           cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];

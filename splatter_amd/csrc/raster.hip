@@ -66,7 +66,7 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   const uint32_t v = blockIdx.y;
   uint32_t* counters = a.counters + (size_t)v * kCounterWords;
   const uint32_t* order = a.order ? a.order + v * a.order_stride : nullptr;
-  const float4* records = a.records + v * a.rec_stride;
+  const uint2* boxes = a.boxes + v * a.box_stride;
   uint32_t* pair_keys = a.pair_keys + v * a.pair_stride;
   uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
   uint64_t* status = a.status + v * a.status_stride;
@@ -92,8 +92,8 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
       cnt[k] = 0;
       if (r < n) {
         const uint32_t c = order ? order[r] : r;
-        const float4 q = records[3 * (size_t)c + 2];
-        const uint32_t lo = __float_as_uint(q.z), hi = __float_as_uint(q.w);
+        const uint2 q = boxes[c];
+        const uint32_t lo = q.x, hi = q.y;
         const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;
         const uint32_t x1 = (hi & 0xFFFFu) / kTileSize, y1 = (hi >> 16) / kTileSize;
         if (lo != 0xFFFFFFFFu && x1 >= x0 && y1 >= y0) {
@@ -149,20 +149,35 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   finish_persistent(&counters[kCtrBin], &counters[kDone + kCtrBin]);
 }
 
+// Run boundaries of the tile-sorted pair keys, four keys per thread.
 __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
   const uint32_t v = blockIdx.y;
   if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
   if (a.clear_b) clear_words(a.clear_b + v * a.clear_b_stride, a.clear_b_words);
-  const uint32_t* keys = a.pair_keys + v * a.pair_stride;
+  const uint32_t* keys = a.pair_keys + v * a.pair_stride;  // 16-byte aligned: pair_stride % 4 == 0
   uint2* ranges = a.ranges + v * a.ranges_stride;
   const uint32_t tiles = a.uniforms[v].tiles_x * a.uniforms[v].tiles_y;
   uint32_t n = a.counters[(size_t)v * kCounterWords + kPairs];
   if (n > a.pair_cap) n = a.pair_cap;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t t = keys[i];
-    if (t >= tiles) continue;
-    if (i == 0 || keys[i - 1] != t) ranges[t].x = i;
-    if (i == n - 1 || keys[i + 1] != t) ranges[t].y = i + 1;
+  const uint32_t step = gridDim.x * blockDim.x * 4;
+  for (uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4; i0 < n; i0 += step) {
+    uint32_t k[6];
+    k[0] = i0 > 0 ? keys[i0 - 1] : 0xFFFFFFFFu;
+    if (i0 + 4 <= n) {
+      const uint4 q = *reinterpret_cast<const uint4*>(keys + i0);
+      k[1] = q.x; k[2] = q.y; k[3] = q.z; k[4] = q.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) k[1 + j] = i0 + j < n ? keys[i0 + j] : 0xFFFFFFFFu;
+    }
+    k[5] = i0 + 4 < n ? keys[i0 + 4] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = i0 + j, t = k[1 + j];
+      if (i >= n || t >= tiles) continue;
+      if (k[j] != t) ranges[t].x = i;
+      if (k[2 + j] != t) ranges[t].y = i + 1;
+    }
   }
 }
 

@@ -67,6 +67,7 @@ struct spl_renderer {
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* compact_idx = nullptr;
   float4* records = nullptr;
+  uint2* boxes = nullptr;
   uint32_t* pair_keys[2] = {nullptr, nullptr};
   uint32_t* pair_vals[2] = {nullptr, nullptr};
   uint32_t* cull_status = nullptr;
@@ -132,6 +133,7 @@ void renderer_free(spl_renderer* r) {
   }
   dev_free(r->compact_idx);
   dev_free(r->records);
+  dev_free(r->boxes);
   dev_free(r->cull_status);
   dev_free(r->tile_prefix);
   dev_free(r->sort_status);
@@ -316,6 +318,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.use_covariance_for_scale = cfg.use_covariance_for_scale ? 1 : 0;
     a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
     a.records = r->records; a.rec_stride = cap * kRecordFloat4;
+    a.boxes = r->boxes; a.box_stride = cap;
     a.conics = b.conics ? r->conics : nullptr;
     if (P > 0) {
       a.clear_a = r->sort_status + (size_t)(P - 1) * r->sort_words;
@@ -336,7 +339,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   {
     BinLaunch a{};
     a.order = sorted ? r->vals[fin] : nullptr; a.order_stride = cap;
-    a.records = r->records; a.rec_stride = cap * kRecordFloat4;
+    a.boxes = r->boxes; a.box_stride = cap;
     a.uniforms = r->d_uniforms;
     a.pair_keys = r->pair_keys[0];
     a.pair_vals = r->pair_vals[0]; a.pair_stride = r->pair_cap;
@@ -392,7 +395,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.clear_b = r->pair_hist;
     a.clear_b_words = (size_t)kMaxPairPasses * 256;
     a.clear_b_stride = (size_t)kMaxPairPasses * 256;
-    a.grid_x = clampg(div_up(r->pair_cap, 256), std::max<uint32_t>(1, cus * 8 / nv));
+    a.grid_x = clampg(div_up(r->pair_cap, 1024), std::max<uint32_t>(1, cus * 32 / nv));
     a.views = (int)nv;
     launch_ranges(a, q);
   }
@@ -552,7 +555,9 @@ int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_rende
   r->depth_bits = (int)cfg->radix_bits_per_digit;
   r->depth_passes = (30 + r->depth_bits - 1) / r->depth_bits;
   const int radix = 1 << r->depth_bits;
-  r->pair_cap = std::min<uint64_t>(r->cap * (uint64_t)r->cfg.tile_pairs_per_splat, (1ull << 30) - (1ull << 20));
+  // a multiple of 4: the tile-range pass reads the pair keys as uint4
+  r->pair_cap = std::min<uint64_t>(r->cap * (uint64_t)r->cfg.tile_pairs_per_splat, (1ull << 30) - (1ull << 20)) & ~3ull;
+  if (r->pair_cap == 0) r->pair_cap = 4;
   r->cull_tiles = div_up(r->cap, kCullTile);
   r->sort_words = (size_t)div_up(r->cap, kSortThreads * kSortItems) * radix;
   r->bin_words = div_up(r->cap, kBinThreads * kBinItems);
@@ -568,6 +573,7 @@ int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_rende
   }
   chk(dev_alloc(&r->compact_idx, V * r->cap));
   chk(dev_alloc(&r->records, V * r->cap * kRecordFloat4));
+  chk(dev_alloc(&r->boxes, V * r->cap));
   chk(dev_alloc(&r->cull_status, V * r->cull_tiles));
   chk(dev_alloc(&r->tile_prefix, V * r->cull_tiles));
   chk(dev_alloc(&r->sort_status, V * r->sort_words * kMaxDepthPasses));

@@ -25,7 +25,7 @@
 namespace spl {
 
 template <int BITS, int THREADS, int ITEMS>
-__global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_onesweep_pass(SortPassLaunch a) {
   constexpr int RADIX = 1 << BITS;
   constexpr int WAVES = THREADS / kWave;
   constexpr int TILE = THREADS * ITEMS;
@@ -85,14 +85,15 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
     const uint32_t tile_base = t * TILE;
     const uint32_t tile_n = (n - tile_base) < (uint32_t)TILE ? (n - tile_base) : (uint32_t)TILE;
 
-    uint32_t key[ITEMS], val[ITEMS], pos[ITEMS];
+    // Register budget: keys die once they are in LDS, values are fetched
+    // while the look-back waits, and the scatter keeps digits packed (four
+    // 8-bit or two 16-bit to a register) instead of 32-bit positions.
+    uint32_t key[ITEMS], pos[ITEMS];
     const uint32_t wave_base = tile_base + w * (kWave * ITEMS) + lane;
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t idx = wave_base + j * kWave;
-      const bool valid = idx < n;
-      key[j] = valid ? keys_in[idx] : 0xFFFFFFFFu;
-      val[j] = valid ? (vals_in ? vals_in[idx] : idx) : 0u;
+      key[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;
     }
 
     // WLMS ranking, items in input order
@@ -145,20 +146,19 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
       }
     }
     uint32_t total;
-    uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+    const uint32_t excl0 = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+    {
+      uint32_t excl = excl0;
 #pragma unroll
-    for (int k = 0; k < DPT; ++k) {
-      const uint32_t d = tid * DPT + k;
-      if (d < (uint32_t)RADIX) {
-        const uint32_t prefix = t == 0 ? 0u : lookback_window<32>(status + d, t, RADIX);
-        store_status(status + (size_t)t * RADIX + d, kFlagInclusive | (prefix + cnt_d[k]));
-        s_local[d] = excl;
-        s_base[d] = s_digit_start[d] + prefix - excl;
+      for (int k = 0; k < DPT; ++k) {
+        const uint32_t d = tid * DPT + k;
+        if (d < (uint32_t)RADIX) s_local[d] = excl;
+        excl += cnt_d[k];
       }
-      excl += cnt_d[k];
     }
     __syncthreads();
 
+    // keys to their place in the tile, sorted by digit
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       if (wave_base + j * kWave < n) {
@@ -167,18 +167,43 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
         if (pos[j] < (uint32_t)TILE) s_buf[pos[j]] = key[j];
       }
     }
+    uint32_t val[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t idx = wave_base + j * kWave;
+      val[j] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0u;
+    }
+    {
+      uint32_t excl = excl0;
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const uint32_t d = tid * DPT + k;
+        if (d < (uint32_t)RADIX) {
+          const uint32_t prefix = t == 0 ? 0u : lookback_window<16>(status + d, t, RADIX);
+          store_status(status + (size_t)t * RADIX + d, kFlagInclusive | (prefix + cnt_d[k]));
+          s_base[d] = s_digit_start[d] + prefix - excl;
+        }
+        excl += cnt_d[k];
+      }
+    }
     __syncthreads();
-    uint32_t out[ITEMS];
+
+    constexpr int PW = BITS <= 8 ? 8 : 16;  // bits per packed digit
+    constexpr int PER = 32 / PW;
+    uint32_t dig[(ITEMS + PER - 1) / PER];
+#pragma unroll
+    for (int k = 0; k < (ITEMS + PER - 1) / PER; ++k) dig[k] = 0u;
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const uint32_t i = k * THREADS + tid;
-      out[k] = 0xFFFFFFFFu;
       if (i < tile_n) {
         const uint32_t kk = s_buf[i];
-        out[k] = s_base[(kk >> shift) & MASK] + i;
+        const uint32_t d = (kk >> shift) & MASK;
+        dig[k / PER] |= d << (PW * (k % PER));
+        const uint32_t o = s_base[d] + i;
         // a position past n means the histogram and the keys disagree:
         // report it instead of storing out of bounds
-        if (out[k] < n) keys_out[out[k]] = kk;
+        if (o < n) keys_out[o] = kk;
         else counters[kScatterError] = 1u;
       }
     }
@@ -190,7 +215,11 @@ __global__ __launch_bounds__(THREADS) void k_onesweep_pass(SortPassLaunch a) {
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const uint32_t i = k * THREADS + tid;
-      if (i < tile_n && out[k] < n) vals_out[out[k]] = s_buf[i];
+      if (i < tile_n) {
+        const uint32_t d = (dig[k / PER] >> (PW * (k % PER))) & ((1u << PW) - 1u);
+        const uint32_t o = s_base[d] + i;
+        if (o < n) vals_out[o] = s_buf[i];
+      }
     }
     __syncthreads();
   }

@@ -114,6 +114,7 @@ struct ProjectLaunch {
   int use_covariance_for_scale;
   int use_unaligned_rectangles;
   float4* records; size_t rec_stride;       // float4 units per view
+  uint2* boxes; size_t box_stride;          // pixel box per entry, dense, for the binning
   double* conics;                           // optional, view 0 only, 6 per entry
   uint32_t* clear_a; size_t clear_a_words; size_t clear_a_stride;
   uint32_t* clear_b; size_t clear_b_words; size_t clear_b_stride;
@@ -122,7 +123,7 @@ struct ProjectLaunch {
 
 struct BinLaunch {
   const uint32_t* order; size_t order_stride;  // record index per draw rank; null = identity
-  const float4* records; size_t rec_stride;
+  const uint2* boxes; size_t box_stride;       // pixel box per record
   const Uniforms* uniforms;
   uint32_t* pair_keys;
   uint32_t* pair_vals; size_t pair_stride;
