@@ -79,6 +79,96 @@ def cpu_baseline(positions: np.ndarray, poses, W: int, H: int, budget_s: float =
                       f"{med * 1e3:.1f} ms/view, {int(np.mean(vis))} visible"}
 
 
+SERVING_SCENES = [300_000, 600_000, 1_000_000, 1_500_000, 2_000_000, 3_000_000, 4_500_000, 6_000_000]
+SERVING_SIZES = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+
+
+def serving(dev, sa, rate: float, seconds: float, warmup_s: float = 1.5, budget_gib: int = 96,
+            in_flight: int = 3, batch: int = 8, seed: int = 7) -> dict:
+    """Poisson stream of render requests over 8 resident clustered scenes
+    (0.3M..6M splats): uniform scene, random pose on the radius-6 sphere,
+    resolution uniform over 640x480..3840x2160.  Latency is the server's
+    submit -> seen-done time of every request that arrived in the measured
+    window; throughput is the frames finished in that window per second."""
+    from splatter_amd.synth import Workload, sphere_pose
+    cfg = sa.Configuration(spherical_harmonics_order=3, max_splat_count=max(SERVING_SCENES), max_views=batch,
+                           tile_pairs_per_splat=8)
+    server = sa.Server(dev, cfg, max_in_flight=in_flight, max_width=3840, max_height=2160,
+                       memory_budget_bytes=budget_gib << 30)
+    t0 = time.perf_counter()
+    scenes = []
+    for i, n in enumerate(SERVING_SCENES):
+        sc = server.create_scene(n)
+        for first, rec in Workload("clustered", n, seed=100 + i).chunks():
+            sc.load_records(first, rec)
+        scenes.append(sc)
+    dev.synchronize()
+    log(f"serving: {len(scenes)} scenes ({sum(SERVING_SCENES) / 1e6:.1f}M splats) resident in {time.perf_counter() - t0:.1f}s")
+    rng = np.random.default_rng(seed)
+    poses = [sphere_pose(10_000 + i) for i in range(512)]
+    lat, sizes, batches = [], [], []
+    done_in_window = 0
+    failed = 0
+    total = warmup_s + seconds
+    start = time.perf_counter()
+    next_t = rng.exponential(1.0 / rate)
+    arrivals = {}
+    while True:
+        now = time.perf_counter() - start
+        while next_t <= now and next_t < total:
+            s = int(rng.integers(len(scenes)))
+            w, h = SERVING_SIZES[int(rng.integers(len(SERVING_SIZES)))]
+            cam, view = poses[int(rng.integers(len(poses)))]
+            t = server.submit(scenes[s], w, h, cam, view)
+            arrivals[t] = next_t
+            next_t += rng.exponential(1.0 / rate)
+        for t in server.poll():
+            info = server.info(t)
+            a = arrivals.pop(t)
+            if info["status"] != 0:
+                failed += 1
+            elif a >= warmup_s:
+                lat.append(info["done_ms"] - info["submit_ms"])
+                sizes.append(info["width"] * info["height"])
+                batches.append(info["batch_views"])
+            if warmup_s <= (time.perf_counter() - start) <= total:
+                done_in_window += 1
+            server.release(t)
+        if now >= total and not arrivals:
+            break
+        if now > total + 30:
+            raise RuntimeError("serving: requests still unfinished 30 s after the last arrival")
+    st = server.stats()
+    server.close()
+    lat = np.array(lat)
+    return {"rate_per_s": rate, "seconds": seconds, "requests": int(len(lat)), "failed": failed,
+            "frames_per_s": round(done_in_window / seconds, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_batch_views": round(float(np.mean(batches)), 2),
+            "memory_peak_gib": round(st["memory_peak"] / 2 ** 30, 2), "memory_budget_gib": budget_gib,
+            "slots": st["slots"], "max_in_flight": in_flight, "batch_width": batch,
+            "scenes_m": [n / 1e6 for n in SERVING_SCENES], "sizes": [f"{w}x{h}" for w, h in SERVING_SIZES]}
+
+
+def copy_calibration(nbytes: int = 256 << 20, repeats: int = 3) -> None:
+    """Device-to-device copies of a known size: under rocprofv3 --pmc their
+    FETCH_SIZE / WRITE_SIZE per dispatch against nbytes give the counters'
+    scale on this device."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed for the copy calibration")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    for _ in range(repeats):
+        hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {repeats} x {nbytes >> 20} MiB device-to-device")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,8 +178,15 @@ def main() -> int:
     ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
-    ap.add_argument("--radix-bits", type=int, default=8)
+    ap.add_argument("--radix-bits", type=int, default=0, help="force the digit width (skips the autotuner)")
+    ap.add_argument("--sort-tile", type=int, default=0, help="force keys per sort tile (skips the autotuner)")
+    ap.add_argument("--no-autotune", action="store_true", help="8-bit digits, 4096-key sort tiles")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrivals per second")
+    ap.add_argument("--serving-seconds", type=float, default=8.0, help="0 skips the serving measurement")
+    ap.add_argument("--profile", action="store_true",
+                    help="batched steps and the stage-timing pass only (every launch has the batch's shape), "
+                         "then three 256 MiB device-to-device copies whose counters scale FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -104,7 +201,7 @@ def main() -> int:
     L = 3
     dev = sa.Device(0)
     cfg = sa.Configuration(spherical_harmonics_order=L, max_splat_count=args.splats, max_views=V_PER,
-                           radix_bits_per_digit=args.radix_bits)
+                           radix_bits_per_digit=args.radix_bits or 8, sort_tile_keys=args.sort_tile or 4096)
     r = sa.Renderer(dev, cfg)
     wl = Workload("clustered", args.splats, seed=2)
     t0 = time.perf_counter()
@@ -115,6 +212,17 @@ def main() -> int:
         positions[first:first + len(rec)] = rec["center"]
     log(f"scene: {args.splats} splats generated+uploaded in {time.perf_counter() - t0:.1f}s")
     poses = orbit_poses(256)
+    tuned = None
+    if not (args.no_autotune or args.radix_bits or args.sort_tile or args.profile):
+        # radix digit width x sort tile (block shape) at this batch width,
+        # each candidate timed on the same resident scene
+        from splatter_amd import autotune as at
+        r.close()
+        tuned = at.autotune(sa, dev, cfg, lambda _r: scene, poses, W, H,
+                            cands=at.candidates(batch_views=(V_PER,)), steps=3, log=log)
+        tuned.apply(cfg)
+        log(f"autotune picked {tuned.best.key()}")
+        r = sa.Renderer(dev, cfg)
     frames = [sa.Frame(dev, W, H) for _ in range(V_PER)]
 
     def batch(k):
@@ -133,6 +241,17 @@ def main() -> int:
     t_batched = time.perf_counter() - t0
     batched_fps = args.steps * V_PER / t_batched
     log(f"batched: {batched_fps:.1f} frames/s")
+
+    if args.profile:
+        r.enable_timing(True)
+        for k in range(args.steps):
+            batch(k)
+        stage_ms = r.stage_times()
+        r.enable_timing(False)
+        copy_calibration()
+        print(json.dumps({"profile": True, "batched_frames_per_s": round(batched_fps, 2), "views_per_batch": V_PER,
+                          "stage_ms_per_frame": {s: round(float(stage_ms[i]), 4) for i, s in enumerate(STAGES)}}))
+        return 0
 
     # plain: the same views, one render_frame each
     n_plain = args.steps * V_PER
@@ -193,6 +312,14 @@ def main() -> int:
     sort_i = STAGES.index("depth_sort")
     sort_gkeys = V / (stage_ms[sort_i] * 1e-3) / 1e9 if stage_ms[sort_i] > 0 else None
 
+    serve = None
+    if args.serving_seconds > 0:
+        r.close()
+        scene.close()
+        serve = serving(dev, sa, args.serving_rate, args.serving_seconds)
+        log(f"serving at {args.serving_rate}/s: p50 {serve['p50_ms']} ms, p99 {serve['p99_ms']} ms, "
+            f"{serve['frames_per_s']} frames/s")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(positions, poses[::32], W, H)
@@ -213,13 +340,18 @@ def main() -> int:
         "config": {"workload": f"{args.splats // 1_000_000}M clustered splats, SH3, {W}x{H}, "
                                f"256-pose orbit, {V_PER} views per batch",
                    "splats": args.splats, "width": W, "height": H, "views_per_batch": V_PER,
-                   "radix_bits_per_digit": cfg.radix_bits_per_digit},
+                   "radix_bits_per_digit": cfg.radix_bits_per_digit, "sort_tile_keys": cfg.sort_tile_keys,
+                   "autotune": tuned.to_json() if tuned else None},
         "plain_frames_per_s": round(plain_fps, 2),
         "batched_over_plain": round(batched_fps / plain_fps, 3),
         "sort_gkeys_per_s": round(sort_gkeys, 2) if sort_gkeys else None,
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
-                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
-                       "kind": "single plain frame, submit to ready, back to back"},
+        "latency_ms": ({"p50": serve["p50_ms"], "p99": serve["p99_ms"],
+                        "kind": f"serving, Poisson {serve['rate_per_s']:g} requests/s, submit to done"}
+                       if serve else None),
+        "plain_frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                                   "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                                   "kind": "single plain frame, submit to ready, back to back"},
+        "serving": serve,
         "visible_per_view": int(V),
         "pairs_per_view": int(K),
         "stages": per_stage,

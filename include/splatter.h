@@ -91,6 +91,7 @@ typedef struct spl_configuration {
     uint32_t tile_pairs_per_splat;        /* capacity of (splat, screen tile) pairs, per max_splat_count */
     uint32_t max_views;                   /* views per batched launch (1..=32); workspace scales with it */
     int32_t use_hip_graphs;               /* replay each batch shape's launch sequence from a hipGraph */
+    uint32_t sort_tile_keys;              /* keys per radix-sort tile: 4096 (256 lanes) or 8192 (512 lanes) */
 } spl_configuration;
 
 const char* spl_last_error(void);
@@ -152,6 +153,70 @@ int spl_render_frame(spl_renderer* renderer, spl_scene* scene, spl_frame* frame,
 int spl_render_views(spl_renderer* renderer, spl_scene* scene, uint32_t view_count,
                      spl_frame* const* frames, const float* camera_matrices,
                      const float* view_matrices);
+
+/* ---- serving form (no counterpart in the reference) ---------------------- */
+/* A request loop over resident scenes: each request is one view of one scene
+ * at its own resolution.  Up to max_in_flight batches run at once, each on a
+ * slot of its own (a renderer with its own stream).  When a slot frees up,
+ * the oldest pending request and the later pending requests of the same
+ * scene (up to renderer.max_views) go out as one spl_render_views batch, so
+ * requests that arrive while others are on the device join the next batch.
+ * Slots (sized per capacity class of the scene: 2^k or 3*2^(k-1) splats) and
+ * frames are pooled within memory_budget_bytes of device memory; idle ones
+ * are released least recently used first, and a batch that does not fit
+ * waits, in arrival order, for running batches to finish.  Scenes are not
+ * counted against the budget.  A request renders exactly what
+ * spl_render_frame renders for the same scene, matrices and size.  All calls
+ * on one server come from one thread. */
+typedef struct spl_server spl_server;
+
+typedef struct spl_server_configuration {
+    spl_configuration renderer;   /* pipeline options of every slot; max_splat_count bounds the scenes,
+                                     max_views is the widest batch */
+    uint32_t max_in_flight;       /* batches on the device at once (1..=8) */
+    uint32_t max_width, max_height; /* largest frame a request may ask for */
+    uint64_t memory_budget_bytes; /* device memory for slot workspaces and frames */
+} spl_server_configuration;
+
+enum { SPL_SERVER_DONE = 0, SPL_SERVER_PENDING = 1, SPL_SERVER_RUNNING = 2 };
+
+typedef struct spl_request_info {
+    int status;                   /* SPL_SERVER_* or a negative error code */
+    uint32_t width, height;
+    uint32_t batch_views;         /* views of the batch the request went out in */
+    double submit_ms, start_ms, done_ms;  /* server clock: submitted, launched, seen done */
+} spl_request_info;
+
+typedef struct spl_server_stats {
+    uint64_t submitted, completed, failed, batches, batched_views, pending;
+    uint32_t slots, slots_busy, frames, reserved;
+    uint64_t memory_in_use, memory_peak, memory_budget;
+} spl_server_stats;
+
+/* max_splat_count 2^23, max_views 8, tile_pairs_per_splat 8, 3 in flight,
+ * 3840x2160, 64 GiB. */
+void spl_server_configuration_default(spl_server_configuration* config);
+int spl_server_create(spl_device* device, const spl_server_configuration* config, spl_server** out);
+/* Waits for running batches and frees every slot and frame. */
+int spl_server_destroy(spl_server* server);
+/* A resident scene for the server's requests (Scene::new without the
+ * renderer's clamp; at most renderer.max_splat_count splats).  Loaded and
+ * destroyed with the spl_scene_* calls; not destroyed while requests use it. */
+int spl_server_scene_create(spl_server* server, uint64_t splat_count, spl_scene** out);
+/* Queues a request and runs the scheduler; *ticket names it from then on. */
+int spl_server_submit(spl_server* server, spl_scene* scene, uint32_t width, uint32_t height,
+                      const float camera_matrix[16], const float view_matrix[16], uint64_t* ticket);
+/* Runs the scheduler (reaps finished batches, admits pending requests) and
+ * returns the tickets finished since the last poll (at most capacity). */
+int spl_server_poll(spl_server* server, uint64_t* finished, uint32_t capacity, uint32_t* count);
+/* Runs the scheduler until the request has finished. */
+int spl_server_wait(spl_server* server, uint64_t ticket);
+int spl_server_request_info(spl_server* server, uint64_t ticket, spl_request_info* info);
+/* Waits for the request and copies its width*height*4 bytes to host memory. */
+int spl_server_read(spl_server* server, uint64_t ticket, void* host, uint64_t size);
+/* Forgets a finished request and returns its frame to the pool. */
+int spl_server_release(spl_server* server, uint64_t ticket);
+int spl_server_get_stats(spl_server* server, spl_server_stats* out);
 
 /* ---- parity taps (replace the commented readbacks, renderer.rs:456-464) - */
 /* The debug reads below describe view `view` (default 0) of the last batch. */

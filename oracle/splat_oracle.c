@@ -273,25 +273,28 @@ void orc_contour_f64(const double scale[3], const double rotation[4], const doub
 /* fragment (shaders.wgsl:580-592) + fixed-function raster and blend       */
 /* ---------------------------------------------------------------------- */
 
-/* e^max(x, -86) for x <= 0 (the fragment's -power/2) by Cody-Waite
- * reduction x = k ln2 + r, |r| <= ln2/2, and the degree-7 Taylor polynomial
- * of e^r in Horner form, scaled by 2^k through the exponent field.  Only IEEE
- * add/mul/floor/max, so it rounds identically on any IEEE-754 machine without
- * contraction.  Below -86 every alpha is far under the 1/255 threshold. */
-float orc_exp(float x) {
-    x = fmaxf(x, -86.0f);
-    float kf = floorf(x * 1.44269502f + 0.5f);
-    float r = (x - kf * 0.693145752f) - kf * 1.42860677e-06f;
-    float p = 0.000198412701f;
-    p = p * r + 0.00138888892f;
-    p = p * r + 0.00833333377f;
-    p = p * r + 0.0416666679f;
-    p = p * r + 0.166666672f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
-    int k = (int)kf;
-    return p * u2f((uint32_t)(k + 127) << 23);
+/* The fragment's Gaussian e^(-|tc|^2 / 2) (shaders.wgsl:586) is evaluated as
+ * 2^-(|S tc|^2) with S = sqrt(log2(e) / 2) folded into each splat's inverse
+ * map (prepare below). */
+#define ORC_GAUSS_SCALE 0.84932180f
+
+/* 2^max(y, -125) for y <= 0: y = k + f with k = rint(y), |f| <= 1/2, the
+ * degree-6 Taylor polynomial of e^(f ln 2) in fused Horner form, then ldexp
+ * by k (exact: the result stays normal).  Only correctly rounded IEEE
+ * operations (fma, rint, ldexp), so it rounds identically on any IEEE-754
+ * machine.  Below -8 every alpha is under the 1/255 threshold. */
+float orc_exp2(float y) {
+    y = fmaxf(y, -125.0f);
+    float k = rintf(y);
+    float f = y - k;
+    float p = 1.54035304e-4f;
+    p = fmaf(p, f, 1.33335581e-3f);
+    p = fmaf(p, f, 9.61812911e-3f);
+    p = fmaf(p, f, 5.55041087e-2f);
+    p = fmaf(p, f, 2.40226507e-1f);
+    p = fmaf(p, f, 6.93147181e-1f);
+    p = fmaf(p, f, 1.0f);
+    return ldexpf(p, (int)k);
 }
 
 static inline uint8_t q8(float x) {
@@ -316,15 +319,15 @@ static void prepare(const orc_record* rec, float margin, int unaligned, orc_prep
     for (int i = 0; i < 6; ++i)
         if (!isfinite(rec->affine[i])) p->valid = 0;
     float det = A00 * A11 - A10 * A01;
-    float rdet = 1.0f / det;
-    p->i00 = A11 * rdet;
-    p->i10 = -A10 * rdet;
-    p->i01 = -A01 * rdet;
-    p->i11 = A00 * rdet;
+    float rs = (1.0f / det) * ORC_GAUSS_SCALE; /* inverse map scaled by S */
+    p->i00 = A11 * rs;
+    p->i10 = -A10 * rs;
+    p->i01 = -A01 * rs;
+    p->i11 = A00 * rs;
     p->tx = rec->affine[4];
     p->ty = rec->affine[5];
     if (unaligned) {
-        p->half = margin;
+        p->half = margin * ORC_GAUSS_SCALE; /* |tc| < margin in scaled coordinates */
     } else {
         float d0 = A00 * A00 + A10 * A10, d1 = A01 * A01 + A11 * A11;
         p->half = sqrtf(fmaxf(d0, d1)) * margin;
@@ -342,32 +345,30 @@ static inline float pixel_qy(int py, float h, float vh) { return (1.0f - (((floa
 static inline void shade(const orc_prepared* p, int unaligned, float qx, float qy,
                          int blend_mode, float* st) {
     float dx = qx - p->tx, dy = qy - p->ty;
-    float tcx = p->i00 * dx + p->i01 * dy;
-    float tcy = p->i10 * dx + p->i11 * dy;
+    float u = fmaf(p->i00, dx, p->i01 * dy); /* S * texture coordinate */
+    float v = fmaf(p->i10, dx, p->i11 * dy);
     float h = p->half;
-    int inside = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
-                           : (dx >= -h && dx < h && dy >= -h && dy < h);
+    int inside = unaligned ? fmaxf(fabsf(u), fabsf(v)) < h : fmaxf(fabsf(dx), fabsf(dy)) < h;
     if (!inside) return;
-    float power = tcx * tcx + tcy * tcy;
-    float alpha = p->a * orc_exp(-0.5f * power);
+    float alpha = p->a * orc_exp2(-fmaf(u, u, v * v));
     if (!(alpha >= 1.0f / 255.0f)) return;
     float sr = p->r * alpha, sg = p->g * alpha, sb = p->b * alpha;
     if (blend_mode == ORC_BLEND_UNORM8) {
         /* st holds the stored unorm8 codes C, T as floats 0..255.  The ideal
-         * unorm blender stores round(255 * (s * T/255 + C/255)) =
-         * round(s * T + C) with s clamped to [0,1] and the sum to 255, and
-         * round(255 * (T/255) * (1 - a)) = round(T * (1 - a)) for alpha. */
+         * unorm blender stores rint(255 * (s * T/255 + C/255)) =
+         * rint(s * T + C) with s clamped to [0,1] and the sum to 255, and
+         * rint(255 * (T/255) * (1 - a)) = rint(T * (1 - a)) for alpha. */
         float t = st[3];
         float sa = fminf(fmaxf(alpha, 0.0f), 1.0f);
         float s[3] = {fminf(fmaxf(sr, 0.0f), 1.0f), fminf(fmaxf(sg, 0.0f), 1.0f), fminf(fmaxf(sb, 0.0f), 1.0f)};
-        for (int c = 0; c < 3; ++c) st[c] = fminf(floorf(s[c] * t + st[c] + 0.5f), 255.0f);
-        st[3] = floorf(t * (1.0f - sa) + 0.5f);
+        for (int c = 0; c < 3; ++c) st[c] = fminf(rintf(fmaf(s[c], t, st[c])), 255.0f);
+        st[3] = rintf(fmaf(-sa, t, t));
     } else {
         float da = st[3];
-        st[0] = sr * da + st[0];
-        st[1] = sg * da + st[1];
-        st[2] = sb * da + st[2];
-        st[3] = da * (1.0f - alpha);
+        st[0] = fmaf(sr, da, st[0]);
+        st[1] = fmaf(sg, da, st[1]);
+        st[2] = fmaf(sb, da, st[2]);
+        st[3] = fmaf(-alpha, da, da);
     }
 }
 

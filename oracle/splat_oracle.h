@@ -22,7 +22,7 @@
  *   orc_project_*              sphere closed forms, silhouette tangency p^T M p = 0,
  *                              known-ellipse round trip, FP32 vs FP64 agreement
  *   orc_sh_color               closed-form constants, quadrature orthonormality
- *   orc_exp                    libm expf within 4 ulp
+ *   orc_exp2                   libm exp2 within 4 ulp over [-125, 0], exact at integers
  *   orc_raster                 isolated-splat analytic footprint, two-layer
  *                              compositing closed form, brute-force bbox check
  * The third-party arithmetic below the reference (geometric_algebra 0.3.0
@@ -125,7 +125,8 @@ void orc_contour_f64(const double scale[3], const double rotation[4], const doub
                      const orc_uniforms* u, double M[9]);
 void orc_sh_color(const float ray_direction[3], const float sh[48], int sh_order, float out[3]);
 float orc_sh_constant(int k);
-float orc_exp(float x);
+/* 2^max(y, -125), y <= 0: the fragment's Gaussian (see orc_raster). */
+float orc_exp2(float y);
 
 /* ---- fixed-function raster + fragment + blend -------------------------- */
 /* records are in draw order (front to back).  image: w*h*4 bytes, channel

@@ -25,11 +25,12 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import SplConfiguration, SplatterError, check, lib
+from ._lib import (SplConfiguration, SplRequestInfo, SplServerConfiguration, SplServerStats, SplatterError,
+                   check, lib)
 
 __all__ = [
     "DepthSorting", "FrameFormat", "BlendMode", "Configuration", "Device", "Renderer",
-    "Scene", "Frame", "SplatterError", "SERIALIZED_SPLAT_DTYPE",
+    "Scene", "Frame", "Server", "SplatterError", "SERIALIZED_SPLAT_DTYPE",
 ]
 
 
@@ -69,6 +70,7 @@ class Configuration:
     tile_pairs_per_splat: int = 4
     max_views: int = 1
     use_hip_graphs: bool = True
+    sort_tile_keys: int = 4096
 
     def to_c(self) -> SplConfiguration:
         c = SplConfiguration()
@@ -87,6 +89,7 @@ class Configuration:
         c.tile_pairs_per_splat = int(self.tile_pairs_per_splat)
         c.max_views = int(self.max_views)
         c.use_hip_graphs = int(bool(self.use_hip_graphs))
+        c.sort_tile_keys = int(self.sort_tile_keys)
         return c
 
 
@@ -307,6 +310,86 @@ class Scene:
     def close(self) -> None:
         if self._h:
             lib().spl_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving form (spl_server_*, include/splatter.h): resident scenes, a
+    request queue admitted by a device-memory budget, continuous batching of
+    same-scene requests over max_in_flight concurrent streams.  A request's
+    image equals Renderer.render_frame's for the same scene, matrices and
+    size.  config=None takes the library's serving defaults (8 views per
+    batch, 8 tile pairs per splat, scenes up to 2^23 splats)."""
+
+    SERVER_DONE, SERVER_PENDING, SERVER_RUNNING = 0, 1, 2
+
+    def __init__(self, device: Device, config: Configuration | None = None, max_in_flight: int = 3,
+                 max_width: int = 3840, max_height: int = 2160, memory_budget_bytes: int = 64 << 30):
+        c = SplServerConfiguration()
+        lib().spl_server_configuration_default(C.byref(c))
+        if config is not None:
+            c.renderer = config.to_c()
+        c.max_in_flight = int(max_in_flight)
+        c.max_width, c.max_height = int(max_width), int(max_height)
+        c.memory_budget_bytes = int(memory_budget_bytes)
+        h = C.c_void_p()
+        check(lib().spl_server_create(device.handle, C.byref(c), C.byref(h)))
+        self.device = device
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def create_scene(self, splat_count: int) -> Scene:
+        h = C.c_void_p()
+        check(lib().spl_server_scene_create(self._h, int(splat_count), C.byref(h)))
+        return Scene(self, int(splat_count), _handle=h)
+
+    def submit(self, scene: Scene, width: int, height: int, camera_matrix, view_matrix) -> int:
+        t = C.c_uint64()
+        check(lib().spl_server_submit(self._h, scene.handle, int(width), int(height), _mat16(camera_matrix),
+                                      _mat16(view_matrix), C.byref(t)))
+        return int(t.value)
+
+    def poll(self, capacity: int = 4096) -> list:
+        buf = (C.c_uint64 * capacity)()
+        n = C.c_uint32()
+        check(lib().spl_server_poll(self._h, buf, capacity, C.byref(n)))
+        return [int(buf[i]) for i in range(n.value)]
+
+    def wait(self, ticket: int) -> None:
+        check(lib().spl_server_wait(self._h, int(ticket)))
+
+    def info(self, ticket: int) -> dict:
+        i = SplRequestInfo()
+        check(lib().spl_server_request_info(self._h, int(ticket), C.byref(i)))
+        return {"status": i.status, "width": i.width, "height": i.height, "batch_views": i.batch_views,
+                "submit_ms": i.submit_ms, "start_ms": i.start_ms, "done_ms": i.done_ms}
+
+    def read(self, ticket: int) -> np.ndarray:
+        info = self.info(ticket)
+        out = np.empty((info["height"], info["width"], 4), np.uint8)
+        check(lib().spl_server_read(self._h, int(ticket), out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def release(self, ticket: int) -> None:
+        check(lib().spl_server_release(self._h, int(ticket)))
+
+    def stats(self) -> dict:
+        st = SplServerStats()
+        check(lib().spl_server_get_stats(self._h, C.byref(st)))
+        return {k: int(getattr(st, k)) for k, _ in SplServerStats._fields_ if k != "reserved"}
+
+    def close(self) -> None:
+        if self._h:
+            lib().spl_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -31,6 +31,53 @@ class SplConfiguration(C.Structure):
         ("tile_pairs_per_splat", C.c_uint32),
         ("max_views", C.c_uint32),
         ("use_hip_graphs", C.c_int32),
+        ("sort_tile_keys", C.c_uint32),
+    ]
+
+
+class SplServerConfiguration(C.Structure):
+    """struct spl_server_configuration (include/splatter.h)."""
+
+    _fields_ = [
+        ("renderer", SplConfiguration),
+        ("max_in_flight", C.c_uint32),
+        ("max_width", C.c_uint32),
+        ("max_height", C.c_uint32),
+        ("memory_budget_bytes", C.c_uint64),
+    ]
+
+
+class SplRequestInfo(C.Structure):
+    """struct spl_request_info (include/splatter.h)."""
+
+    _fields_ = [
+        ("status", C.c_int),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("batch_views", C.c_uint32),
+        ("submit_ms", C.c_double),
+        ("start_ms", C.c_double),
+        ("done_ms", C.c_double),
+    ]
+
+
+class SplServerStats(C.Structure):
+    """struct spl_server_stats (include/splatter.h)."""
+
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("failed", C.c_uint64),
+        ("batches", C.c_uint64),
+        ("batched_views", C.c_uint64),
+        ("pending", C.c_uint64),
+        ("slots", C.c_uint32),
+        ("slots_busy", C.c_uint32),
+        ("frames", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("memory_in_use", C.c_uint64),
+        ("memory_peak", C.c_uint64),
+        ("memory_budget", C.c_uint64),
     ]
 
 
@@ -59,6 +106,17 @@ SIGNATURES = [
     ("spl_frame_read", C.c_int, [_P, _P, C.c_uint64]),
     ("spl_render_frame", C.c_int, [_P, _P, _P, _F16, _F16]),
     ("spl_render_views", C.c_int, [_P, _P, C.c_uint32, _PP, _F16, _F16]),
+    ("spl_server_configuration_default", None, [C.POINTER(SplServerConfiguration)]),
+    ("spl_server_create", C.c_int, [_P, C.POINTER(SplServerConfiguration), _PP]),
+    ("spl_server_destroy", C.c_int, [_P]),
+    ("spl_server_scene_create", C.c_int, [_P, C.c_uint64, _PP]),
+    ("spl_server_submit", C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _F16, _F16, C.POINTER(C.c_uint64)]),
+    ("spl_server_poll", C.c_int, [_P, C.POINTER(C.c_uint64), C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("spl_server_wait", C.c_int, [_P, C.c_uint64]),
+    ("spl_server_request_info", C.c_int, [_P, C.c_uint64, C.POINTER(SplRequestInfo)]),
+    ("spl_server_read", C.c_int, [_P, C.c_uint64, _P, C.c_uint64]),
+    ("spl_server_release", C.c_int, [_P, C.c_uint64]),
+    ("spl_server_get_stats", C.c_int, [_P, C.POINTER(SplServerStats)]),
     ("spl_debug_select_view", C.c_int, [_P, C.c_uint32]),
     ("spl_debug_visible_count", C.c_int, [_P, C.POINTER(C.c_uint64)]),
     ("spl_debug_read_entries", C.c_int, [_P, _P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -1,0 +1,121 @@
+"""Autotuner for the batched form: picks the radix digit width, the radix-sort
+tile (keys per tile, i.e. the sort block's lane count) and the batch width
+(views per launch sequence) for one scene, resolution and pose set by timing
+every candidate on the device, and can keep the answer in a JSON cache.
+
+The candidates all render identical images and sort orders (the tests check
+each option against the oracle), so the tuner is free to pick on speed alone.
+"""
+from __future__ import annotations
+
+import itertools
+import json
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+RADIX_BITS = (8, 10)
+SORT_TILE_KEYS = (4096, 8192)
+BATCH_VIEWS = (8, 16, 32)
+
+
+@dataclass
+class Candidate:
+    radix_bits_per_digit: int
+    sort_tile_keys: int
+    max_views: int
+
+    def key(self) -> str:
+        return f"r{self.radix_bits_per_digit}-t{self.sort_tile_keys}-v{self.max_views}"
+
+
+@dataclass
+class TuneResult:
+    best: Candidate
+    frames_per_s: float
+    table: list = field(default_factory=list)  # (candidate key, frames/s)
+
+    def apply(self, config) -> None:
+        config.radix_bits_per_digit = self.best.radix_bits_per_digit
+        config.sort_tile_keys = self.best.sort_tile_keys
+        config.max_views = self.best.max_views
+
+    def to_json(self) -> dict:
+        return {"best": self.best.__dict__, "frames_per_s": self.frames_per_s, "table": self.table}
+
+    @classmethod
+    def from_json(cls, d: dict) -> "TuneResult":
+        return cls(Candidate(**d["best"]), float(d["frames_per_s"]), [tuple(x) for x in d.get("table", [])])
+
+
+def candidates(radix_bits=RADIX_BITS, sort_tile_keys=SORT_TILE_KEYS, batch_views=BATCH_VIEWS):
+    return [Candidate(r, t, v) for r, t, v in itertools.product(radix_bits, sort_tile_keys, batch_views)]
+
+
+def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand: Candidate,
+            steps: int = 3) -> float:
+    """frames/s of `steps` graph-replayed batches of cand.max_views views."""
+    cfg = type(base)(**{k: getattr(base, k) for k in base.__dataclass_fields__})
+    cfg.radix_bits_per_digit = cand.radix_bits_per_digit
+    cfg.sort_tile_keys = cand.sort_tile_keys
+    cfg.max_views = cand.max_views
+    r = sa.Renderer(device, cfg)
+    try:
+        scene = scene_loader(r)
+        frames = [sa.Frame(device, width, height) for _ in range(cand.max_views)]
+        nv = cand.max_views
+
+        def batch(k):
+            sel = [poses[(k * nv + i) % len(poses)] for i in range(nv)]
+            r.render_views(frames, [p[0] for p in sel], [p[1] for p in sel], scene)
+
+        batch(0)
+        device.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            batch(k + 1)
+        device.synchronize()
+        return steps * nv / (time.perf_counter() - t0)
+    finally:
+        r.close()
+
+
+def autotune(sa, device, base, scene_loader, poses, width: int, height: int, cands=None, steps: int = 3,
+             log=None) -> TuneResult:
+    """Times every candidate; scene_loader(renderer) returns the scene to
+    render (it may return the same resident scene every time)."""
+    cands = cands or candidates()
+    table = []
+    for c in cands:
+        fps = measure(sa, device, base, scene_loader, poses, width, height, c, steps)
+        table.append((c.key(), round(fps, 2)))
+        if log:
+            log(f"autotune {c.key()}: {fps:.1f} frames/s")
+    best_i = int(np.argmax([t[1] for t in table]))
+    return TuneResult(cands[best_i], table[best_i][1], table)
+
+
+def cache_key(device_name: str, splats: int, width: int, height: int, sh_order: int) -> str:
+    bucket = 1 << max(0, int(splats - 1).bit_length())
+    return f"{device_name}|n<={bucket}|{width}x{height}|sh{sh_order}"
+
+
+def load_cached(path: str, key: str):
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return TuneResult.from_json(d[key]) if key in d else None
+
+
+def store_cached(path: str, key: str, result: TuneResult) -> None:
+    d = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            d = json.load(f)
+    d[key] = result.to_json()
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)

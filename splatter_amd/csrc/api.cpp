@@ -53,6 +53,7 @@ void spl_configuration_default(spl_configuration* c) {
   c->tile_pairs_per_splat = 4;
   c->max_views = 1;
   c->use_hip_graphs = 1;
+  c->sort_tile_keys = 4096;
 }
 
 int spl_device_open(int ordinal, spl_device** out) {
@@ -98,13 +99,19 @@ int spl_synchronize(spl_device* d) {
 
 int spl_scene_create(spl_renderer* r, uint64_t splat_count, spl_scene** out) {
   if (!r || !out) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_scene_create: null argument");
+  return scene_create(renderer_device(r), std::min<uint64_t>(splat_count, renderer_capacity(r)), out);  // scene.rs:46
+}
+
+}  // extern "C"
+
+int spl::scene_create(spl_device* dev, uint64_t count, spl_scene** out) {
   *out = nullptr;
-  spl_device* dev = renderer_device(r);
+  if (count > (1ull << 30)) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_scene_create: more than 2^30 splats");
   HIP_TRY(hipSetDevice(dev->ordinal));
   auto* s = new (std::nothrow) spl_scene();
   if (!s) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_scene_create: host allocation");
   s->dev = dev;
-  s->count = std::min<uint64_t>(splat_count, renderer_capacity(r));  // scene.rs:46
+  s->count = count;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
   const uint64_t slots = std::max<uint64_t>(s->count, 1);
@@ -118,6 +125,9 @@ int spl_scene_create(spl_renderer* r, uint64_t splat_count, spl_scene** out) {
     chk(hipMemset(s->rotations, 0, slots * 16));
     chk(hipMemset(s->scale_alpha, 0, slots * 16));
     chk(hipMemset(s->sh, 0, slots * 192));
+    // the uploads go to the device's (non-blocking) stream: finish the
+    // clears before any of them can start
+    chk(hipDeviceSynchronize());
   }
   if (e != hipSuccess) {
     dev_free(s->centers);
@@ -131,6 +141,8 @@ int spl_scene_create(spl_renderer* r, uint64_t splat_count, spl_scene** out) {
   *out = s;
   return SPL_OK;
 }
+
+extern "C" {
 
 int spl_scene_splat_count(const spl_scene* s, uint64_t* out) {
   if (!s || !out) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_scene_splat_count: null argument");

@@ -20,24 +20,29 @@ namespace spl {
 
 namespace {
 
-// e^max(x, -86) for x <= 0 (the fragment's -power/2): Cody-Waite reduction
-// x = k ln2 + r, |r| <= ln2/2, degree-7 Taylor polynomial of e^r in Horner
-// form, 2^k through the exponent field.  Only IEEE add/mul/floor/max, so it
-// rounds the same on the CPU and the GPU.  Below -86 every alpha is far under
-// the 1/255 fragment threshold, so the clamp changes no pixel.
-__device__ __forceinline__ float spl_exp(float x) {
-  x = fmaxf(x, -86.0f);
-  const float kf = floorf(x * 1.44269502f + 0.5f);
-  const float r = (x - kf * 0.693145752f) - kf * 1.42860677e-06f;
-  float p = 0.000198412701f;
-  p = p * r + 0.00138888892f;
-  p = p * r + 0.00833333377f;
-  p = p * r + 0.0416666679f;
-  p = p * r + 0.166666672f;
-  p = p * r + 0.5f;
-  p = p * r + 1.0f;
-  p = p * r + 1.0f;
-  return p * __int_as_float((int)(kf + 127.0f) << 23);
+// The fragment's Gaussian e^(-|tc|^2 / 2) (shaders.wgsl:586) is evaluated as
+// 2^-(|S tc|^2) with S = sqrt(log2(e) / 2) folded into each entry's inverse
+// map, so a pixel costs two fused products per coordinate and one exp2.
+constexpr float kGaussScale = 0.84932180f;
+
+// 2^max(y, -125) for y <= 0: y = k + f with k = rint(y), |f| <= 1/2, the
+// degree-6 Taylor polynomial of e^(f ln 2) in fused Horner form, then
+// ldexp by k (exact: the result stays normal).  Only correctly rounded IEEE
+// operations (fma, rint, ldexp), so it rounds the same on the CPU and the
+// GPU.  Below -8 every alpha is already under the 1/255 fragment threshold,
+// so the clamp changes no pixel.
+__device__ __forceinline__ float spl_exp2(float y) {
+  y = fmaxf(y, -125.0f);
+  const float k = rintf(y);
+  const float f = y - k;
+  float p = 1.54035304e-4f;
+  p = fmaf(p, f, 1.33335581e-3f);
+  p = fmaf(p, f, 9.61812911e-3f);
+  p = fmaf(p, f, 5.55041087e-2f);
+  p = fmaf(p, f, 2.40226507e-1f);
+  p = fmaf(p, f, 6.93147181e-1f);
+  p = fmaf(p, f, 1.0f);
+  return ldexpf(p, (int)k);
 }
 
 __device__ __forceinline__ float q8(float x) {
@@ -189,6 +194,7 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
 // then keeps, in order, only the entries whose pixel box meets its quadrant
 // (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
 // stops blending, and the block stops when all four have.
+template <bool UNALIGNED, bool F32>
 __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   constexpr int B = 256;
   __shared__ float4 s_m[2][B];     // i00 i01 i10 i11
@@ -227,8 +233,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
                                   (1.0f - (((float)qy1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
   const float2 q_hi = make_float2(((((float)qx1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
                                   (1.0f - (((float)qy0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
-  const bool f32 = a.blend_f32 != 0;
-  const bool unaligned = a.use_unaligned_rectangles != 0;
+  constexpr bool f32 = F32;
+  constexpr bool unaligned = UNALIGNED;
   const float margin = u.ellipse_margin;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = f32 ? 1.0f : 255.0f;
   const uint32_t visible = a.counters[(size_t)v * kCounterWords + kVisible];
@@ -247,9 +253,12 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   auto stage = [&](int buf) {
     const float A00 = g1.x, A10 = g1.y, A01 = g1.z, A11 = g1.w;
     const float det = A00 * A11 - A10 * A01;
-    const float rdet = 1.0f / det;
-    s_m[buf][tid] = make_float4(A11 * rdet, -A01 * rdet, -A10 * rdet, A00 * rdet);
-    const float half = unaligned ? margin : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
+    const float rs = (1.0f / det) * kGaussScale;  // inverse map scaled by S
+    s_m[buf][tid] = make_float4(A11 * rs, -A01 * rs, -A10 * rs, A00 * rs);
+    // unaligned: the rectangle |tc| < margin in scaled coordinates;
+    // axis-aligned: a square of half-side radius * margin in the view plane
+    const float half = unaligned ? margin * kGaussScale
+                                 : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
     s_t[buf][tid] = make_float4(g2.x, g2.y, half, g0.w);
     s_c[buf][tid] = make_float4(g0.x, g0.y, g0.z, 0.0f);
     s_box[buf][tid] = make_uint2(__float_as_uint(g2.z), __float_as_uint(g2.w));
@@ -287,7 +296,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
             const float4 t = s_t[buf][j];
             const float dxl = q_lo.x - t.x, dxh = q_hi.x - t.x, dyl = q_lo.y - t.y, dyh = q_hi.y - t.y;
             float ax0, ax1, ay0, ay1;  // range of the tested coordinate pair
-            if (unaligned) {
+            if constexpr (unaligned) {
               ax0 = fminf(m.x * dxl, m.x * dxh) + fminf(m.y * dyl, m.y * dyh);
               ax1 = fmaxf(m.x * dxl, m.x * dxh) + fmaxf(m.y * dyl, m.y * dyh);
               ay0 = fminf(m.z * dxl, m.z * dxh) + fminf(m.w * dyl, m.w * dyh);
@@ -322,30 +331,28 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
         for (int k = 0; k < 4; ++k) {
           if (i + k >= n) break;
           const float dx = qx - t[k].x, dy = qy - t[k].y;
-          const float tcx = m[k].x * dx + m[k].y * dy;
-          const float tcy = m[k].z * dx + m[k].w * dy;
+          const float u = fmaf(m[k].x, dx, m[k].y * dy);  // S * texture coordinate
+          const float w2 = fmaf(m[k].z, dx, m[k].w * dy);
           const float h = t[k].z;
-          const bool in = unaligned ? (tcx >= -h && tcx < h && tcy >= -h && tcy < h)
-                                    : (dx >= -h && dx < h && dy >= -h && dy < h);
+          const bool in = unaligned ? fmaxf(fabsf(u), fabsf(w2)) < h : fmaxf(fabsf(dx), fabsf(dy)) < h;
           if (in && tr != 0.0f) {
-            const float power = tcx * tcx + tcy * tcy;
-            const float alpha = t[k].w * spl_exp(-0.5f * power);
+            const float alpha = t[k].w * spl_exp2(-fmaf(u, u, w2 * w2));
             if (alpha >= 1.0f / 255.0f) {
               const float sr = col[k].x * alpha, sg = col[k].y * alpha, sb = col[k].z * alpha;
-              if (!f32) {
-                // unorm8 codes C, T: round(s * T + C) (s clamped, sum capped at 255)
-                // and round(T * (1 - a)): the ideal 8-bit blender, no division
+              if constexpr (!f32) {
+                // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
+                // 255) and rint(T * (1 - a)): the ideal 8-bit blender
                 const float sa = clamp01(alpha);
-                c0 = fminf(floorf(clamp01(sr) * tr + c0 + 0.5f), 255.0f);
-                c1 = fminf(floorf(clamp01(sg) * tr + c1 + 0.5f), 255.0f);
-                c2 = fminf(floorf(clamp01(sb) * tr + c2 + 0.5f), 255.0f);
-                tr = floorf(tr * (1.0f - sa) + 0.5f);
+                c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
+                c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
+                c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
+                tr = rintf(fmaf(-sa, tr, tr));
               } else {
                 const float da = tr;
-                c0 = sr * da + c0;
-                c1 = sg * da + c1;
-                c2 = sb * da + c2;
-                tr = da * (1.0f - alpha);
+                c0 = fmaf(sr, da, c0);
+                c1 = fmaf(sg, da, c1);
+                c2 = fmaf(sb, da, c2);
+                tr = fmaf(-alpha, da, da);
               }
             }
           }
@@ -379,7 +386,14 @@ void launch_ranges(const RangesLaunch& a, hipStream_t s) {
 }
 
 void launch_raster(const RasterLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_raster, dim3(a.grid_x, a.views), dim3(256), 0, s, a);
+  const dim3 grid(a.grid_x, a.views);
+  if (a.use_unaligned_rectangles) {
+    if (a.blend_f32) hipLaunchKernelGGL((k_raster<true, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_raster<true, false>), grid, dim3(256), 0, s, a);
+  } else {
+    if (a.blend_f32) hipLaunchKernelGGL((k_raster<false, true>), grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_raster<false, false>), grid, dim3(256), 0, s, a);
+  }
 }
 
 }  // namespace spl

@@ -45,6 +45,14 @@ struct BatchShape {
   }
 };
 
+// Instantiated launch sequences, one per batch shape seen recently.
+struct GraphEntry {
+  BatchShape shape;
+  hipGraphExec_t exec = nullptr;
+  uint64_t used = 0;
+};
+constexpr size_t kGraphCache = 16;
+
 int pair_passes_for(uint32_t tiles) {
   if (tiles <= 256) return 1;
   if (tiles <= 65536) return 2;
@@ -59,6 +67,7 @@ struct spl_renderer {
   uint64_t cap = 0;
   uint32_t max_views = 1;
   int depth_bits = 8, depth_passes = 4;
+  int sort_threads = kSortThreads;  // lanes per radix-sort tile
   uint64_t pair_cap = 0;
   // per-view sizes (words)
   size_t cull_tiles = 0, sort_words = 0, bin_words = 0, pair_words = 0;
@@ -106,8 +115,13 @@ struct spl_renderer {
   double stage_sum_ms[kStageCount] = {};
   uint64_t stage_frames = 0;
   // replay of the launch sequence
-  hipGraphExec_t graph = nullptr;
-  BatchShape graph_shape;
+  std::vector<GraphEntry> graphs;
+  uint64_t graph_clock = 0;
+  // the stream every launch and copy of this renderer goes to: the device's,
+  // or one of its own (a server slot)
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint64_t bytes = 0;  // device memory held
 };
 
 namespace spl {
@@ -118,9 +132,8 @@ uint64_t renderer_capacity(const spl_renderer* r) { return r->cap; }
 namespace {
 
 void drop_graph(spl_renderer* r) {
-  if (r->graph) (void)hipGraphExecDestroy(r->graph);
-  r->graph = nullptr;
-  r->graph_shape = BatchShape{};
+  for (auto& g : r->graphs) (void)hipGraphExecDestroy(g.exec);
+  r->graphs.clear();
 }
 
 void renderer_free(spl_renderer* r) {
@@ -160,6 +173,9 @@ void renderer_free(spl_renderer* r) {
   r->ev_pool.clear();
   r->ev_views.clear();
   r->ev_used = 0;
+  if (r->own_stream && r->stream) (void)hipStreamDestroy(r->stream);
+  r->stream = nullptr;
+  r->own_stream = false;
 }
 
 // utils.rs:58-65
@@ -204,21 +220,23 @@ void frame_uniforms(const spl_configuration& cfg, const float cam[16], const flo
 
 int ensure_ranges(spl_renderer* r, uint32_t tiles) {
   if (tiles <= r->ranges_cap) return SPL_OK;
-  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
   drop_graph(r);
   dev_free(r->ranges);
+  r->bytes -= (uint64_t)r->ranges_cap * r->max_views * sizeof(uint2);
   r->ranges_cap = 0;
   const size_t words = (size_t)tiles * r->max_views;
   HIP_TRY(dev_alloc(&r->ranges, words));
-  HIP_TRY(hipMemset(r->ranges, 0, words * sizeof(uint2)));
-  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemsetAsync(r->ranges, 0, words * sizeof(uint2), r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
   r->ranges_cap = tiles;
+  r->bytes += (uint64_t)words * sizeof(uint2);
   return SPL_OK;
 }
 
 int ensure_tile_work(spl_renderer* r, uint32_t tiles) {
   if (!r->tile_stats_on || r->tile_work_cap >= tiles) return SPL_OK;
-  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
   drop_graph(r);
   dev_free(r->tile_work);
   r->tile_work_cap = 0;
@@ -242,6 +260,9 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   const int Q = b.pair_passes;
   const uint32_t cus = (uint32_t)r->dev->cu_count;
   const uint32_t per_view = std::max<uint32_t>(1, cus * 4 / nv);
+  // radix-sort tiles: two 512-lane or four 256-lane blocks per CU
+  const uint32_t sort_tile = (uint32_t)r->sort_threads * kSortItems;
+  const uint32_t sort_per_view = std::max<uint32_t>(1, cus * 4 * kSortThreads / r->sort_threads / nv);
   const size_t cap = r->cap;
   const size_t sort_stride = r->sort_words * kMaxDepthPasses;
   const size_t pair_stride_status = r->pair_words * kMaxPairPasses;
@@ -301,7 +322,8 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a_words = sort_used;
       a.clear_a_stride = sort_stride;
     }
-    a.grid_x = clampg(div_up(n, kSortThreads * kSortItems), per_view);
+    a.threads = r->sort_threads;
+    a.grid_x = clampg(div_up(n, sort_tile), sort_per_view);
     a.views = (int)nv;
     launch_sort_pass(bits, a, q);
   }
@@ -376,7 +398,8 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a_words = r->pair_words;
       a.clear_a_stride = pair_stride_status;
     }
-    a.grid_x = clampg(div_up(r->pair_cap, kSortThreads * kSortItems), per_view);
+    a.threads = r->sort_threads;
+    a.grid_x = clampg(div_up(r->pair_cap, sort_tile), sort_per_view);
     a.views = (int)nv;
     launch_sort_pass(8, a, q);
   }
@@ -422,7 +445,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
 // One batch of at most max_views views.
 int enqueue_batch(spl_renderer* r, spl_scene* s, uint32_t nv, spl_frame* const* frames, const float* cams,
                   const float* views) {
-  hipStream_t q = r->dev->stream;
+  hipStream_t q = r->stream;
   Uniforms u[kMaxViews];
   uint32_t max_tiles = 1;
   for (uint32_t v = 0; v < nv; ++v) {
@@ -467,22 +490,31 @@ int enqueue_batch(spl_renderer* r, spl_scene* s, uint32_t nv, spl_frame* const* 
     r->ev_views[r->ev_used] = nv;
     launch_sequence(r, b, q, r->ev_pool[r->ev_used++].data());
   } else if (r->cfg.use_hip_graphs) {
-    if (!r->graph || !(r->graph_shape == b)) {
-      drop_graph(r);
+    GraphEntry* hit = nullptr;
+    for (auto& g : r->graphs)
+      if (g.shape == b) hit = &g;
+    if (!hit) {
+      if (r->graphs.size() >= kGraphCache) {
+        auto lru = std::min_element(r->graphs.begin(), r->graphs.end(),
+                                    [](const GraphEntry& x, const GraphEntry& y) { return x.used < y.used; });
+        (void)hipGraphExecDestroy(lru->exec);
+        r->graphs.erase(lru);
+      }
       hipGraph_t g = nullptr;
       HIP_TRY(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
       launch_sequence(r, b, q, nullptr);
       hipError_t e = hipStreamEndCapture(q, &g);
       if (e != hipSuccess) return fail(SPL_ERROR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
+      GraphEntry entry;
+      entry.shape = b;
+      e = hipGraphInstantiate(&entry.exec, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
-      if (e != hipSuccess) {
-        r->graph = nullptr;
-        return fail(SPL_ERROR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-      }
-      r->graph_shape = b;
+      if (e != hipSuccess) return fail(SPL_ERROR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+      r->graphs.push_back(entry);
+      hit = &r->graphs.back();
     }
-    HIP_TRY(hipGraphLaunch(r->graph, q));
+    hit->used = ++r->graph_clock;
+    HIP_TRY(hipGraphLaunch(hit->exec, q));
   } else {
     launch_sequence(r, b, q, nullptr);
   }
@@ -497,7 +529,7 @@ int enqueue_batch(spl_renderer* r, spl_scene* s, uint32_t nv, spl_frame* const* 
 
 int read_counter(spl_renderer* r, uint32_t view, int slot, uint32_t* v) {
   HIP_TRY(hipSetDevice(r->dev->ordinal));
-  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
   HIP_TRY(hipMemcpy(v, r->counters + (size_t)view * kCounterWords + slot, 4, hipMemcpyDeviceToHost));
   return SPL_OK;
 }
@@ -526,11 +558,13 @@ int draw_order(spl_renderer* r, std::vector<uint32_t>& order, std::vector<uint32
 
 }  // namespace
 
-extern "C" {
+namespace spl {
 
-int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_renderer** out) {
-  if (!dev || !cfg || !out) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_create: null argument");
-  *out = nullptr;
+int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stream, uint32_t presize_tiles,
+                    spl_renderer** out, uint64_t* dry_bytes) {
+  if (!dev || !cfg || (!out && !dry_bytes)) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_create: null argument");
+  if (out) *out = nullptr;
+  const bool dry = dry_bytes != nullptr;
   if (cfg->spherical_harmonics_order > 3)
     return fail(SPL_ERROR_INVALID_ARGUMENT, "spherical_harmonics_order must be 0..=3");
   if (cfg->radix_bits_per_digit != 8 && cfg->radix_bits_per_digit != 10)
@@ -543,17 +577,30 @@ int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_rende
     return fail(SPL_ERROR_INVALID_ARGUMENT, "field_of_view_y must be in (0, pi)");
   if (cfg->max_views > (uint32_t)kMaxViews)
     return fail(SPL_ERROR_INVALID_ARGUMENT, "max_views must be in 1..=32");
+  if (cfg->sort_tile_keys != 0 && cfg->sort_tile_keys != 4096 && cfg->sort_tile_keys != 8192)
+    return fail(SPL_ERROR_UNSUPPORTED, "sort_tile_keys must be 4096 or 8192");
   HIP_TRY(hipSetDevice(dev->ordinal));
   auto* r = new (std::nothrow) spl_renderer();
   if (!r) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: host allocation");
   r->dev = dev;
   r->cfg = *cfg;
+  r->stream = dev->stream;
+  if (own_stream && !dry) {
+    hipError_t se = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    if (se != hipSuccess) {
+      delete r;
+      return fail(SPL_ERROR_DEVICE, std::string("spl_renderer_create: stream: ") + hipGetErrorString(se));
+    }
+    r->own_stream = true;
+  }
   if (r->cfg.tile_pairs_per_splat == 0) r->cfg.tile_pairs_per_splat = 4;
   if (r->cfg.max_views == 0) r->cfg.max_views = 1;
   r->max_views = r->cfg.max_views;
   r->cap = cfg->max_splat_count;
   r->depth_bits = (int)cfg->radix_bits_per_digit;
   r->depth_passes = (30 + r->depth_bits - 1) / r->depth_bits;
+  if (r->cfg.sort_tile_keys == 0) r->cfg.sort_tile_keys = 4096;
+  r->sort_threads = r->cfg.sort_tile_keys == 8192 ? kSortThreadsWide : kSortThreads;
   const int radix = 1 << r->depth_bits;
   // a multiple of 4: the tile-range pass reads the pair keys as uint4
   r->pair_cap = std::min<uint64_t>(r->cap * (uint64_t)r->cfg.tile_pairs_per_splat, (1ull << 30) - (1ull << 20)) & ~3ull;
@@ -565,26 +612,35 @@ int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_rende
   const size_t V = r->max_views;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  auto alloc = [&](auto** p, size_t count) {
+    r->bytes += std::max<size_t>(count, 1) * sizeof(**p);
+    return dry ? hipSuccess : dev_alloc(p, count);
+  };
   for (int i = 0; i < 2; ++i) {
-    chk(dev_alloc(&r->keys[i], V * r->cap));
-    chk(dev_alloc(&r->vals[i], V * r->cap));
-    chk(dev_alloc(&r->pair_keys[i], V * r->pair_cap));
-    chk(dev_alloc(&r->pair_vals[i], V * r->pair_cap));
+    chk(alloc(&r->keys[i], V * r->cap));
+    chk(alloc(&r->vals[i], V * r->cap));
+    chk(alloc(&r->pair_keys[i], V * r->pair_cap));
+    chk(alloc(&r->pair_vals[i], V * r->pair_cap));
   }
-  chk(dev_alloc(&r->compact_idx, V * r->cap));
-  chk(dev_alloc(&r->records, V * r->cap * kRecordFloat4));
-  chk(dev_alloc(&r->boxes, V * r->cap));
-  chk(dev_alloc(&r->cull_status, V * r->cull_tiles));
-  chk(dev_alloc(&r->tile_prefix, V * r->cull_tiles));
-  chk(dev_alloc(&r->sort_status, V * r->sort_words * kMaxDepthPasses));
-  chk(dev_alloc(&r->bin_status, V * r->bin_words));
-  chk(dev_alloc(&r->pair_status, V * r->pair_words * kMaxPairPasses));
-  chk(dev_alloc(&r->counters, V * kCounterWords));
-  chk(dev_alloc(&r->depth_hist, V * kMaxDepthPasses * 1024));
-  chk(dev_alloc(&r->pair_hist, V * kMaxPairPasses * 256));
-  chk(dev_alloc(&r->vis, r->cap));
-  chk(dev_alloc(&r->d_uniforms, V));
-  chk(dev_alloc(&r->d_pixels, V));
+  chk(alloc(&r->compact_idx, V * r->cap));
+  chk(alloc(&r->records, V * r->cap * kRecordFloat4));
+  chk(alloc(&r->boxes, V * r->cap));
+  chk(alloc(&r->cull_status, V * r->cull_tiles));
+  chk(alloc(&r->tile_prefix, V * r->cull_tiles));
+  chk(alloc(&r->sort_status, V * r->sort_words * kMaxDepthPasses));
+  chk(alloc(&r->bin_status, V * r->bin_words));
+  chk(alloc(&r->pair_status, V * r->pair_words * kMaxPairPasses));
+  chk(alloc(&r->counters, V * kCounterWords));
+  chk(alloc(&r->depth_hist, V * kMaxDepthPasses * 1024));
+  chk(alloc(&r->pair_hist, V * kMaxPairPasses * 256));
+  chk(alloc(&r->vis, r->cap));
+  chk(alloc(&r->d_uniforms, V));
+  chk(alloc(&r->d_pixels, V));
+  if (dry) {
+    *dry_bytes = r->bytes + (uint64_t)presize_tiles * V * sizeof(uint2);
+    delete r;
+    return SPL_OK;
+  }
   for (auto& sl : r->slots) {
     chk(hipHostMalloc((void**)&sl.uniforms, V * sizeof(Uniforms), hipHostMallocDefault));
     chk(hipHostMalloc((void**)&sl.pixels, V * sizeof(uint32_t*), hipHostMallocDefault));
@@ -593,14 +649,14 @@ int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_rende
   if (e == hipSuccess) {
     // Every stage clears the bookkeeping of the stage before it once that
     // stage is done, so this is the only initialisation ever needed.
-    chk(hipMemset(r->cull_status, 0, V * r->cull_tiles * 4));
-    chk(hipMemset(r->sort_status, 0, V * r->sort_words * kMaxDepthPasses * 4));
-    chk(hipMemset(r->bin_status, 0, V * r->bin_words * 8));
-    chk(hipMemset(r->pair_status, 0, V * r->pair_words * kMaxPairPasses * 4));
-    chk(hipMemset(r->counters, 0, V * kCounterWords * 4));
-    chk(hipMemset(r->depth_hist, 0, V * kMaxDepthPasses * 1024 * 4));
-    chk(hipMemset(r->pair_hist, 0, V * kMaxPairPasses * 256 * 4));
-    chk(hipDeviceSynchronize());
+    chk(hipMemsetAsync(r->cull_status, 0, V * r->cull_tiles * 4, r->stream));
+    chk(hipMemsetAsync(r->sort_status, 0, V * r->sort_words * kMaxDepthPasses * 4, r->stream));
+    chk(hipMemsetAsync(r->bin_status, 0, V * r->bin_words * 8, r->stream));
+    chk(hipMemsetAsync(r->pair_status, 0, V * r->pair_words * kMaxPairPasses * 4, r->stream));
+    chk(hipMemsetAsync(r->counters, 0, V * kCounterWords * 4, r->stream));
+    chk(hipMemsetAsync(r->depth_hist, 0, V * kMaxDepthPasses * 1024 * 4, r->stream));
+    chk(hipMemsetAsync(r->pair_hist, 0, V * kMaxPairPasses * 256 * 4, r->stream));
+    chk(hipStreamSynchronize(r->stream));
   }
   if (e != hipSuccess) {
     renderer_free(r);
@@ -609,14 +665,39 @@ int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_rende
       return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: device memory for max_splat_count x max_views");
     return fail(SPL_ERROR_DEVICE, std::string("spl_renderer_create: ") + hipGetErrorString(e));
   }
+  if (presize_tiles) {
+    const int rc = ensure_ranges(r, presize_tiles);
+    if (rc) {
+      renderer_free(r);
+      delete r;
+      return rc;
+    }
+  }
   *out = r;
   return SPL_OK;
+}
+
+hipStream_t renderer_stream(const spl_renderer* r) { return r->stream; }
+uint64_t renderer_bytes(const spl_renderer* r) { return r->bytes; }
+
+int renderer_counter_words(const spl_renderer* r, uint32_t view, const uint32_t** words) {
+  if (view >= r->max_views) return fail(SPL_ERROR_INVALID_ARGUMENT, "view outside the renderer's batch width");
+  *words = r->counters + (size_t)view * kCounterWords;
+  return SPL_OK;
+}
+
+}  // namespace spl
+
+extern "C" {
+
+int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_renderer** out) {
+  return renderer_create(dev, cfg, false, 0, out, nullptr);
 }
 
 int spl_renderer_destroy(spl_renderer* r) {
   if (!r) return SPL_OK;
   (void)hipSetDevice(r->dev->ordinal);
-  (void)hipStreamSynchronize(r->dev->stream);
+  (void)hipStreamSynchronize(r->stream);
   renderer_free(r);
   delete r;
   return SPL_OK;
@@ -755,7 +836,7 @@ int spl_debug_read_tile_stats(spl_renderer* r, uint32_t* out, uint64_t capacity,
   if (!r->tile_stats_on || !r->tile_work || !r->have_frame)
     return fail(SPL_ERROR_INVALID_ARGUMENT, "tile statistics not enabled");
   HIP_TRY(hipSetDevice(r->dev->ordinal));
-  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
   *count = r->last_tiles[0];
   const size_t m = (size_t)std::min<uint64_t>(capacity, r->last_tiles[0]);
   if (out && m) HIP_TRY(hipMemcpy(out, r->tile_work, m * sizeof(uint2), hipMemcpyDeviceToHost));
@@ -765,7 +846,7 @@ int spl_debug_read_tile_stats(spl_renderer* r, uint32_t* out, uint64_t capacity,
 int spl_debug_enable_timing(spl_renderer* r, int enable) {
   if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
   HIP_TRY(hipSetDevice(r->dev->ordinal));
-  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
   r->timing = enable != 0;
   r->ev_used = 0;
   for (double& v : r->stage_sum_ms) v = 0.0;
@@ -776,7 +857,7 @@ int spl_debug_enable_timing(spl_renderer* r, int enable) {
 int spl_debug_stage_times(spl_renderer* r, float* ms, uint32_t capacity) {
   if (!r || !ms) return fail(SPL_ERROR_INVALID_ARGUMENT, "null argument");
   HIP_TRY(hipSetDevice(r->dev->ordinal));
-  HIP_TRY(hipStreamSynchronize(r->dev->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
   for (size_t f = 0; f < r->ev_used; ++f) {
     for (int i = 0; i < kStageCount; ++i) {
       float t = 0.0f;

@@ -8,7 +8,7 @@
 //    bumped by the lowest peer) carries the rank across the wave's items.
 //    Items are visited in input order, so the pass is stable -- the
 //    reference ranks with LDS atomicAdd (shaders.wgsl:428-429), which is not.
-//  - tiles are TILE = THREADS*ITEMS keys (4096) taken in order from an
+//  - tiles are TILE = THREADS*ITEMS keys (4096 or 8192) taken in order from an
 //    atomic counter by a persistent grid (reference: 1024-key tiles, one
 //    workgroup each, shaders.wgsl:406-420).
 //  - the global digit offsets come from the upsweep histogram and are
@@ -227,10 +227,18 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
 }
 
 void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s) {
-  if (bits == 10)
-    hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreads, kSortItems>), dim3(a.grid_x, a.views), dim3(kSortThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreads, kSortItems>), dim3(a.grid_x, a.views), dim3(kSortThreads), 0, s, a);
+  const dim3 grid(a.grid_x, a.views);
+  if (a.threads == kSortThreadsWide) {
+    if (bits == 10)
+      hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreadsWide, kSortItems>), grid, dim3(kSortThreadsWide), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreadsWide, kSortItems>), grid, dim3(kSortThreadsWide), 0, s, a);
+  } else {
+    if (bits == 10)
+      hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreads, kSortItems>), grid, dim3(kSortThreads), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreads, kSortItems>), grid, dim3(kSortThreads), 0, s, a);
+  }
 }
 
 }  // namespace spl

@@ -17,6 +17,21 @@ int fail(int code, const std::string& msg);
 spl_device* renderer_device(const spl_renderer* r);
 uint64_t renderer_capacity(const spl_renderer* r);
 
+// spl_renderer_create with a stream of the renderer's own (instead of the
+// device's) and the tile-range buffer sized up front for `presize_tiles`
+// screen tiles, so no later frame up to that size allocates.  With
+// dry_bytes set nothing is allocated: the device bytes such a renderer would
+// hold are written there instead.
+int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stream, uint32_t presize_tiles,
+                    spl_renderer** out, uint64_t* dry_bytes);
+hipStream_t renderer_stream(const spl_renderer* r);
+uint64_t renderer_bytes(const spl_renderer* r);  // device memory the renderer holds
+// Device address of view `view`'s counter block (kCounterWords u32).
+int renderer_counter_words(const spl_renderer* r, uint32_t view, const uint32_t** words);
+
+// Scene::new without a renderer: `count` splats, zeroed, on `dev`.
+int scene_create(spl_device* dev, uint64_t count, spl_scene** out);
+
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 template <typename T>

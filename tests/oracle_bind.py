@@ -60,8 +60,8 @@ def lib():
         l.orc_sh_color.argtypes = [P, P, C.c_int, P]
         l.orc_sh_constant.argtypes = [C.c_int]
         l.orc_sh_constant.restype = C.c_float
-        l.orc_exp.argtypes = [C.c_float]
-        l.orc_exp.restype = C.c_float
+        l.orc_exp2.argtypes = [C.c_float]
+        l.orc_exp2.restype = C.c_float
         l.orc_raster.argtypes = [P, C.c_size_t, C.POINTER(Uniforms), C.c_int, C.c_int, P, C.c_int]
         l.orc_render_frame.argtypes = [P, C.c_size_t, C.POINTER(Uniforms), C.c_int, C.c_int, C.c_int,
                                        C.c_int, P, P, P, P]
@@ -155,5 +155,5 @@ def render_frame(splats: np.ndarray, u: Uniforms, sh_order: int, covariance: boo
     return keys[:v].copy(), idx[:v].copy(), rec[:v].copy(), img
 
 
-def exp(x: float) -> float:
-    return lib().orc_exp(float(x))
+def exp2(y: float) -> float:
+    return lib().orc_exp2(float(y))

@@ -111,9 +111,12 @@ def test_depth_sorting_none_draws_in_index_order():
     check_case(case, conics=False)
 
 
-def test_radix_10_bit_digits():
-    case = Case(Workload("gaussian", 200_000, seed=5), config(radix_bits_per_digit=10), 800, 600,
-                sphere_pose(3))
+@pytest.mark.parametrize("bits,tile", [(10, 4096), (8, 8192), (10, 8192)])
+def test_radix_digit_width_and_sort_tile(bits, tile):
+    # every option the autotuner may pick sorts to the same bit-exact order
+    say(f"radix bits {bits}, sort tile {tile}")
+    case = Case(Workload("gaussian", 200_000, seed=5), config(radix_bits_per_digit=bits, sort_tile_keys=tile),
+                800, 600, sphere_pose(3))
     check_case(case, conics=False)
 
 

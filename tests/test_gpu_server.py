@@ -1,0 +1,160 @@
+"""Serving form (spl_server_*): requests of several resident scenes and
+resolutions admitted by a memory budget and batched while others run.  Each
+request's image must be the same in the mix as alone, and the same as the
+plain single-frame path's (which the parity tests hold to the oracle)."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import splatter_amd as sa
+from gpu_common import config, device, say
+from splatter_amd.synth import Workload, sphere_pose
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(320, 240), (640, 480), (200, 150), (640, 360)]
+SCENES = [("gaussian", 30_000, 11), ("clustered", 60_000, 12), ("gaussian", 100_000, 13)]
+
+
+def requests(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = int(rng.integers(len(SCENES)))
+        w, h = SIZES[int(rng.integers(len(SIZES)))]
+        out.append((s, w, h, sphere_pose(seed * 1000 + i)))
+    return out
+
+
+def server_config():
+    return config(max_splat_count=1 << 17, max_views=4, tile_pairs_per_splat=8)
+
+
+def make_scenes(server):
+    scenes, records = [], []
+    for kind, n, seed in SCENES:
+        rec = Workload(kind, n, seed=seed).all()
+        sc = server.create_scene(n)
+        sc.load_records(0, rec)
+        scenes.append(sc)
+        records.append(rec)
+    device().synchronize()
+    return scenes, records
+
+
+def plain_images(reqs, records):
+    cfg = config(max_splat_count=1 << 17)
+    r = sa.Renderer(device(), cfg)
+    scenes = []
+    for (kind, n, seed), rec in zip(SCENES, records):
+        sc = sa.Scene(r, n)
+        sc.load_records(0, rec)
+        scenes.append(sc)
+    out = []
+    for s, w, h, (cam, view) in reqs:
+        f = sa.Frame(device(), w, h)
+        r.render_frame(f, cam, view, scenes[s])
+        out.append(f.read())
+    return out
+
+
+def test_server_mix_equals_alone_and_plain():
+    say("server: 3 scenes, 16 mixed requests at once, then each alone")
+    server = sa.Server(device(), server_config(), max_in_flight=3, max_width=640, max_height=480,
+                       memory_budget_bytes=2 << 30)
+    scenes, records = make_scenes(server)
+    reqs = requests(16, seed=5)
+    tickets = [server.submit(scenes[s], w, h, cam, view) for s, w, h, (cam, view) in reqs]
+    mixed = []
+    for t in tickets:
+        server.wait(t)
+        info = server.info(t)
+        assert info["status"] == sa.Server.SERVER_DONE
+        assert info["submit_ms"] <= info["start_ms"] <= info["done_ms"]
+        mixed.append(server.read(t))
+        server.release(t)
+    st = server.stats()
+    say("stats after the mix:", st)
+    assert st["completed"] == 16 and st["failed"] == 0
+    assert st["batches"] < 16  # same-scene requests went out together
+    assert st["memory_peak"] <= st["memory_budget"]
+    alone = []
+    for s, w, h, (cam, view) in reqs:
+        t = server.submit(scenes[s], w, h, cam, view)
+        alone.append(server.read(t))
+        assert server.info(t)["batch_views"] == 1
+        server.release(t)
+    plain = plain_images(reqs, records)
+    for i, (a, b, c) in enumerate(zip(mixed, alone, plain)):
+        assert a.shape == (reqs[i][2], reqs[i][1], 4)
+        assert np.array_equal(a, b), f"request {i}: mixed and alone differ"
+        assert np.array_equal(a, c), f"request {i}: server and render_frame differ"
+    server.close()
+
+
+def test_server_against_oracle():
+    say("server: one request per scene against the oracle")
+    server = sa.Server(device(), server_config(), max_in_flight=2, max_width=640, max_height=480,
+                       memory_budget_bytes=1 << 30)
+    scenes, records = make_scenes(server)
+    cfg = server_config()
+    for s, (kind, n, seed) in enumerate(SCENES):
+        cam, view = sphere_pose(77 + s)
+        w, h = SIZES[s]
+        t = server.submit(scenes[s], w, h, cam, view)
+        img = server.read(t)[..., [2, 1, 0, 3]]  # BGRA target
+        server.release(t)
+        u = ob.uniforms(cam, view, w, h, fov=cfg.field_of_view_y, tol=cfg.frustum_culling_tolerance,
+                        margin=cfg.ellipse_margin, scale=cfg.splat_scale)
+        _, _, _, want = ob.render_frame(ob.decode(records[s]), u, sh_order=3,
+                                        covariance=cfg.use_covariance_for_scale)
+        d = np.abs(img.astype(np.int16) - want.astype(np.int16)).max()
+        say(f"scene {s}: max diff {d}/255")
+        assert d <= 2
+    server.close()
+
+
+def test_server_tight_budget_serialises_but_completes():
+    say("server: budget for one slot at a time")
+    cfg = server_config()
+    probe = sa.Server(device(), cfg, max_in_flight=3, max_width=640, max_height=480, memory_budget_bytes=8 << 30)
+    scenes, records = make_scenes(probe)
+    reqs = requests(10, seed=9)
+    # learn one slot's size for the largest scene, then allow little more
+    t = probe.submit(scenes[2], 640, 480, *reqs[0][3])
+    probe.wait(t)
+    slot_bytes = probe.stats()["memory_in_use"] - 640 * 480 * 4
+    want = [probe.read(t)]
+    probe.release(t)
+    for s, w, h, (cam, view) in reqs[1:]:
+        t = probe.submit(scenes[s], w, h, cam, view)
+        want.append(probe.read(t))
+        probe.release(t)
+    probe.close()
+    budget = slot_bytes + 4 * 640 * 480 * 4 + (1 << 20)
+    server = sa.Server(device(), cfg, max_in_flight=3, max_width=640, max_height=480, memory_budget_bytes=budget)
+    scenes2, _ = make_scenes(server)
+    reqs[0] = (2, 640, 480, reqs[0][3])
+    tickets = [server.submit(scenes2[s], w, h, cam, view) for s, w, h, (cam, view) in reqs]
+    for i, t in enumerate(tickets):
+        img = server.read(t)
+        assert np.array_equal(img, want[i]), f"request {i}"
+        server.release(t)
+    st = server.stats()
+    say("stats under the tight budget:", st)
+    assert st["memory_peak"] <= budget and st["completed"] == len(reqs)
+    server.close()
+
+
+def test_server_rejects_what_cannot_fit():
+    server = sa.Server(device(), server_config(), max_in_flight=1, max_width=320, max_height=240,
+                       memory_budget_bytes=1 << 20)
+    sc = server.create_scene(1000)
+    sc.load_records(0, Workload("gaussian", 1000, seed=3).all())
+    cam, view = sphere_pose(1)
+    with pytest.raises(sa.SplatterError):
+        server.submit(sc, 640, 480, cam, view)  # larger than max_width x max_height
+    t = server.submit(sc, 320, 240, cam, view)
+    with pytest.raises(sa.SplatterError):
+        server.wait(t)  # no slot fits in 1 MiB
+    server.close()

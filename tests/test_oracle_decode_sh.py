@@ -1,5 +1,5 @@
 """Pins of the oracle's record decode (scene.rs:225-249), spherical harmonics
-(shaders.wgsl:248-294) and exp (used by the fragment, shaders.wgsl:586)."""
+(shaders.wgsl:248-294) and exp2 (the fragment's Gaussian, shaders.wgsl:586)."""
 import math
 
 import numpy as np
@@ -89,15 +89,28 @@ def test_sh_order_zero_special_case():
         assert np.array_equal(col, want.astype(np.float32))
 
 
-def test_exp_against_libm():
-    # the fragment's exp clamps at -86 (e^-86 * alpha is far below 1/255)
-    xs = np.concatenate([np.linspace(-86.0, 0, 200001, dtype=np.float32),
-                         np.float32([-0.5, -1e-3, -1e-7, 0.0, -4.0])])
-    got = np.array([ob.exp(float(x)) for x in xs[::7]], np.float32)
-    want = np.exp(xs[::7].astype(np.float64)).astype(np.float32)
+def test_exp2_against_libm():
+    # the fragment's Gaussian is 2^-(|S tc|^2); exp2 clamps at -125 (far below
+    # any alpha of 1/255) and must be exact where 2^y is a power of two
+    ys = np.concatenate([np.linspace(-125.0, 0, 250001, dtype=np.float32),
+                         np.float32([-0.5, -1e-3, -1e-7, 0.0, -4.0, -0.49999997, -0.50000006])])
+    got = np.array([ob.exp2(float(y)) for y in ys[::7]], np.float32)
+    want = np.exp2(ys[::7].astype(np.float64)).astype(np.float32)
     ulp = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
     assert ulp.max() <= 4
-    assert ob.exp(-100.0) == ob.exp(-86.0) < 1e-37 and ob.exp(0.0) == 1.0
+    for k in range(0, 126):
+        assert ob.exp2(-float(k)) == 2.0 ** -k
+    assert ob.exp2(-200.0) == ob.exp2(-125.0) == 2.0 ** -125
+
+
+def test_exp2_gaussian_scale():
+    # 2^-((S x)^2) with S = sqrt(log2(e)/2) is e^(-x^2/2)
+    S = np.float32(0.84932180)
+    assert abs(float(S) ** 2 - math.log2(math.e) / 2) < 1e-7
+    for x in (0.0, 0.5, 1.0, 1.7, 2.0, 2.8):
+        u = np.float32(S * np.float32(x))
+        got = ob.exp2(-float(np.float32(u * u)))
+        assert abs(got - math.exp(-x * x / 2)) <= 2e-6 * max(1e-3, math.exp(-x * x / 2)) + 1e-7
 
 
 def test_serialized_dtype_matches_reference_layout():
