@@ -11,7 +11,11 @@
 // The conic and its decomposition run in FP64: the contour path's semi-axes
 // (shaders.wgsl:217-237) lose most of their digits in FP32 to the
 // cancellation in e = M.zz - t^T M t.  Products and sums follow the WGSL's
-// order with no contraction (-ffp-contract=off).
+// order; in FP64 a product and the sum it feeds are fused (one rounding
+// instead of two, 2^-53 relative) and divisions by the same value share one
+// reciprocal, which keeps them far inside the 1e-5 conic tolerance.  The
+// f32 colour path is not contracted: it matches the reference's f32
+// arithmetic operation for operation.
 #include "spl_device.h"
 #include "spl_internal.h"
 
@@ -21,6 +25,7 @@ namespace {
 
 // WGSL column-major 3x3: m[c][r].  (X Y)[c][r] = sum_k X[k][r] Y[c][k].
 __device__ __forceinline__ void mat3_mul(const double X[3][3], const double Y[3][3], double out[3][3]) {
+#pragma clang fp contract(fast)
   double tmp[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -48,6 +53,7 @@ __device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.
 
 // shaders.wgsl:63-79 quatToMat
 __device__ __forceinline__ void quat_to_mat(const double p[4], double m[3][3]) {
+#pragma clang fp contract(fast)
   const double s2 = sqrt(2.0);
   const double q0 = p[0] * s2, q1 = p[1] * s2, q2 = p[2] * s2, q3 = p[3] * s2;
   const double yy = q1 * q1, yz = q1 * q2, yw = q1 * q3, yx = q1 * q0;
@@ -63,13 +69,16 @@ __device__ __forceinline__ void quat_to_mat(const double p[4], double m[3][3]) {
 __device__ __forceinline__ void contour_transform(const double scale[3], const double rot[4], double T[3][3]) {
   quat_to_mat(rot, T);
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
+  for (int c = 0; c < 3; ++c) {
+    const double inv = 1.0 / scale[c];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] / scale[c];
+    for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] * inv;
+  }
 }
 
 // shaders.wgsl:125-169 projectedContourOfEllipsoid, given the transform.
 __device__ void contour(const double T0[3][3], const double pos[3], const Uniforms& u, double M[3][3]) {
+#pragma clang fp contract(fast)
   double C[3][3], T[3][3], A[3][3], Ct[3][3], Tt[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -109,6 +118,7 @@ __device__ __forceinline__ void covariance_transform(const double scale[3], cons
 
 // shaders.wgsl:99-118 projectedCovarianceOfEllipsoid, given the transform.
 __device__ void covariance(const double T[3][3], const double pos[3], const Uniforms& u, double cov[3][3]) {
+#pragma clang fp contract(fast)
   double C[3][3], J[3][3], Tt[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -120,13 +130,14 @@ __device__ void covariance(const double T[3][3], const double pos[3], const Unif
   for (int r = 0; r < 4; ++r)
     vp[r] = (((double)V[r] * pos[0] + (double)V[4 + r] * pos[1]) + (double)V[8 + r] * pos[2]) +
             (double)V[12 + r] * 1.0;
-  double qx = vp[0] / vp[2], qy = vp[1] / vp[2];
+  const double iz = 1.0 / vp[2];
+  double qx = vp[0] * iz, qy = vp[1] * iz;
   qx = qx < -1.0 ? -1.0 : (qx > 1.0 ? 1.0 : qx);
   qy = qy < -1.0 ? -1.0 : (qy > 1.0 ? 1.0 : qy);
   vp[0] = qx * vp[2];
   vp[1] = qy * vp[2];
-  J[0][0] = 1.0 / vp[2]; J[0][1] = 0.0; J[0][2] = -vp[0] / (vp[2] * vp[2]);
-  J[1][0] = 0.0; J[1][1] = 1.0 / vp[2]; J[1][2] = -vp[1] / (vp[2] * vp[2]);
+  J[0][0] = iz; J[0][1] = 0.0; J[0][2] = -vp[0] * (iz * iz);
+  J[1][0] = 0.0; J[1][1] = iz; J[1][2] = -vp[1] * (iz * iz);
   J[2][0] = 0.0; J[2][1] = 0.0; J[2][2] = 0.0;
   mat3_transpose(T, Tt);
   mat3_mul(Tt, C, Tt);
@@ -311,11 +322,12 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
         const double pos[3] = {(double)cx, (double)cy, (double)cz};
         const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
         double M[3][3];
+        float A00, A10, A01, A11, tx, ty;
         {
+#pragma clang fp contract(fast)
           double Tc[3][3];
           contour_transform(scale, rot, Tc);
           contour(Tc, pos, u, M);
-        }
         // extractTranslationOfEllipse (shaders.wgsl:179-191)
         const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
         const double inv_disc = 1.0 / disc;
@@ -349,9 +361,10 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
         }
         const double bias = (double)u.ellipse_size_bias;
         const double sa0 = bias + s0, sb0 = bias + s1;
-        const float A00 = (float)(kk * sa0), A10 = (float)(-jj * sa0);
-        const float A01 = (float)(jj * sb0), A11 = (float)(kk * sb0);
-        const float tx = (float)t0, ty = (float)t1;
+        A00 = (float)(kk * sa0); A10 = (float)(-jj * sa0);
+        A01 = (float)(jj * sb0); A11 = (float)(kk * sb0);
+        tx = (float)t0; ty = (float)t1;
+        }
         uint32_t box_lo, box_hi;
         pixel_box(A00, A10, A01, A11, tx, ty, u, a.use_unaligned_rectangles != 0, box_lo, box_hi);
         float4* rec = a.records + v * a.rec_stride + 3 * (size_t)c;
