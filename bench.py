@@ -53,6 +53,26 @@ def stage_bytes(V: float, K: float, N: int, W: int, H: int, L: int, P: int, Q: i
     }
 
 
+STAGE_KERNELS = {"cull_key_compact": "k_cull_views", "digit_histograms": "k_digit_histograms",
+                 "depth_sort": "k_onesweep_pass", "project": "k_project", "bin": "k_bin",
+                 "pair_sort": "k_onesweep_pass", "tile_ranges": "k_tile_ranges", "raster": "k_raster"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_6M_clustered_1080p_16views.txt")
+
+
+def measured_traffic(stage: str):
+    """HBM bytes per launch of the stage's kernel (FETCH_SIZE + WRITE_SIZE,
+    scaled against a device copy of known size) from the committed rocprofv3
+    summary of this workload, or None when there is none."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    want = STAGE_KERNELS.get(stage)
+    for line in open(TRAFFIC_FILE):
+        f = line.split()
+        if len(f) >= 4 and not line.startswith("#") and f[0].startswith(want) and f[0] != "kernel":
+            return int(float(f[2]) + float(f[3]))
+    return None
+
+
 def cpu_baseline(positions: np.ndarray, poses, W: int, H: int, budget_s: float = 12.0) -> dict:
     """The reference's CPU depth-sort fallback (renderer.rs:372-396) restated
     in C, one core, on the same scene: median over views."""
@@ -358,7 +378,7 @@ def main() -> int:
         "raster_tiles": tiles_info,
         "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_batch": launches,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

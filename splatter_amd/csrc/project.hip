@@ -228,17 +228,21 @@ __device__ void pixel_box(float A00, float A10, float A01, float A11, float tx, 
 // One record per (visible splat, view).  Entry c of view v's list (its c-th
 // visible splat by index) is found from the visibility masks and the cull's
 // per-tile prefixes, so records land where the cull put the keys, with no
-// per-view index to read.  For each group of 256 splats of a tile the block
-// lists its (splat, view) pairs with a scan and every lane projects one pair,
-// so lanes do not idle on views that do not see their splat; the lanes of
-// one splat read its attributes from the same cache lines.
+// per-view index to read.  The block lists the tile's (splat, view) pairs
+// group by group (256 splats each, a scan of the mask popcounts) together
+// with each pair's entry c, and projects the list once it holds as many
+// pairs as LDS allows: every lane projects one pair, lanes do not idle on
+// views that do not see their splat even when a frame has one view, and the
+// lanes of one splat read its attributes from the same cache lines.
 template <int L>
 __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
   constexpr int THREADS = kCullThreads, ITEMS = kCullItems, WAVES = THREADS / kWave, TILE = kCullTile;
+  constexpr int CAP = 4096;  // listed pairs per round (one group of 256 splats has at most 256 * kMaxViews)
+  static_assert(THREADS * kMaxViews <= 2 * CAP, "a group's pairs fit after one flush");
   __shared__ uint32_t s_off[kMaxViews][ITEMS * WAVES];
   __shared__ uint32_t s_tp[kMaxViews];
-  __shared__ uint64_t s_bal[kMaxViews][WAVES];
-  __shared__ uint16_t s_items[THREADS * kMaxViews];
+  __shared__ uint32_t s_item[2 * CAP];  // slot in tile << 5 | view
+  __shared__ uint32_t s_entry[2 * CAP]; // entry c of that view
   __shared__ uint32_t s_tmp[WAVES];
   const uint32_t nv = a.views;
   for (uint32_t v = 0; v < nv; ++v) {
@@ -252,82 +256,46 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
   constexpr int SH_F4 = (SH_FLOATS + 3) / 4;
   const double splat_scale = (double)a.uniforms[0].splat_scale;
 
-  for (uint32_t t = blockIdx.x; t < num_tiles; t += gridDim.x) {
-    // per view: exclusive offsets of the (item, wave) groups inside the tile
-    for (uint32_t v = 0; v < nv; ++v) {
+  // projects listed pairs [0, count) of tile t
+  auto project_pairs = [&](uint32_t t, uint32_t count) {
+    for (uint32_t k = tid; k < count; k += THREADS) {
+      const uint32_t it = s_item[k];
+      const uint32_t v = it & 31u;
+      const uint32_t i = t * TILE + (it >> 5);
+      const uint32_t c = s_entry[k];
+      const Uniforms& u = a.uniforms[v];
+
+      const float cx = a.scene.centers[3 * (size_t)i + 0];
+      const float cy = a.scene.centers[3 * (size_t)i + 1];
+      const float cz = a.scene.centers[3 * (size_t)i + 2];
+      const float4 rq = a.scene.rotations[i];
+      const float4 sa = a.scene.scale_alpha[i];
+      float sh[SH_F4 * 4];
+      const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t idx = t * TILE + j * THREADS + tid;
-        const uint32_t m = idx < n ? a.vis[idx] : 0u;
-        const uint64_t b = __ballot((m >> v) & 1u);
-        if (lane == 0) s_off[v][j * WAVES + w] = (uint32_t)__popcll(b);
+      for (int q = 0; q < SH_F4; ++q) {
+        const float4 x = shp[q];
+        sh[4 * q + 0] = x.x; sh[4 * q + 1] = x.y; sh[4 * q + 2] = x.z; sh[4 * q + 3] = x.w;
       }
-    }
-    __syncthreads();
-    for (uint32_t v = w; v < nv; v += WAVES) {
-      const uint32_t c = s_off[v][lane];
-      uint32_t incl = c;
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o, kWave);
-        if (lane >= (uint32_t)o) incl += y;
-      }
-      s_off[v][lane] = incl - c;
-      if (lane == 0) s_tp[v] = a.tile_prefix[v * a.tp_stride + t];
-    }
-    __syncthreads();
 
-    for (int j = 0; j < ITEMS; ++j) {
-      const uint32_t base = t * TILE + j * THREADS;
-      const uint32_t mine = base + tid < n ? a.vis[base + tid] : 0u;
-      for (uint32_t v = 0; v < nv; ++v) {
-        const uint64_t b = __ballot((mine >> v) & 1u);
-        if (lane == 0) s_bal[v][w] = b;
-      }
-      uint32_t total;
-      uint32_t slot = block_exclusive_scan<THREADS>((uint32_t)__popc(mine), s_tmp, total);
-      for (uint32_t m = mine; m; m &= m - 1) s_items[slot++] = (uint16_t)((tid << 5) | (uint32_t)__ffs(m) - 1u);
-      __syncthreads();
-      for (uint32_t k = tid; k < total; k += THREADS) {
-        const uint32_t it = s_items[k];
-        const uint32_t sl = it >> 5, v = it & 31u;
-        const uint32_t i = base + sl;
-        const uint32_t sw = sl / kWave, sbit = sl % kWave;
-        const uint32_t c = s_tp[v] + s_off[v][j * WAVES + sw] +
-                           (uint32_t)__popcll(s_bal[v][sw] & ((1ull << sbit) - 1ull));
-        const Uniforms& u = a.uniforms[v];
+      // shaders.wgsl:531-532: colour from the normalised ray direction
+      float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
+      const float len = sqrtf((rd[0] * rd[0] + rd[1] * rd[1]) + rd[2] * rd[2]);
+      rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
+      float rgb[3];
+      sh_color<L>(rd, sh, rgb);
 
-        const float cx = a.scene.centers[3 * (size_t)i + 0];
-        const float cy = a.scene.centers[3 * (size_t)i + 1];
-        const float cz = a.scene.centers[3 * (size_t)i + 2];
-        const float4 rq = a.scene.rotations[i];
-        const float4 sa = a.scene.scale_alpha[i];
-        float sh[SH_F4 * 4];
-        const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
-#pragma unroll
-        for (int q = 0; q < SH_F4; ++q) {
-          const float4 x = shp[q];
-          sh[4 * q + 0] = x.x; sh[4 * q + 1] = x.y; sh[4 * q + 2] = x.z; sh[4 * q + 3] = x.w;
-        }
-
-        // shaders.wgsl:531-532: colour from the normalised ray direction
-        float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
-        const float len = sqrtf((rd[0] * rd[0] + rd[1] * rd[1]) + rd[2] * rd[2]);
-        rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
-        float rgb[3];
-        sh_color<L>(rd, sh, rgb);
-
-        // shaders.wgsl:533-547
-        const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
-        const double pos[3] = {(double)cx, (double)cy, (double)cz};
-        const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
-        double M[3][3];
-        float A00, A10, A01, A11, tx, ty;
-        {
+      // shaders.wgsl:533-547
+      const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
+      const double pos[3] = {(double)cx, (double)cy, (double)cz};
+      const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
+      double M[3][3];
+      float A00, A10, A01, A11, tx, ty;
+      {
 #pragma clang fp contract(fast)
-          double Tc[3][3];
-          contour_transform(scale, rot, Tc);
-          contour(Tc, pos, u, M);
+        double Tc[3][3];
+        contour_transform(scale, rot, Tc);
+        contour(Tc, pos, u, M);
         // extractTranslationOfEllipse (shaders.wgsl:179-191)
         const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
         const double inv_disc = 1.0 / disc;
@@ -364,22 +332,73 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
         A00 = (float)(kk * sa0); A10 = (float)(-jj * sa0);
         A01 = (float)(jj * sb0); A11 = (float)(kk * sb0);
         tx = (float)t0; ty = (float)t1;
-        }
-        uint32_t box_lo, box_hi;
-        pixel_box(A00, A10, A01, A11, tx, ty, u, a.use_unaligned_rectangles != 0, box_lo, box_hi);
-        float4* rec = a.records + v * a.rec_stride + 3 * (size_t)c;
-        rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
-        rec[1] = make_float4(A00, A10, A01, A11);
-        rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
-        a.boxes[v * a.box_stride + c] = make_uint2(box_lo, box_hi);
-        if (a.conics && v == 0) {
-          double* cn = a.conics + 6 * (size_t)c;
-          cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];
-          cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
+      }
+      uint32_t box_lo, box_hi;
+      pixel_box(A00, A10, A01, A11, tx, ty, u, a.use_unaligned_rectangles != 0, box_lo, box_hi);
+      float4* rec = a.records + v * a.rec_stride + 3 * (size_t)c;
+      rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
+      rec[1] = make_float4(A00, A10, A01, A11);
+      rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
+      a.boxes[v * a.box_stride + c] = make_uint2(box_lo, box_hi);
+      if (a.conics && v == 0) {
+        double* cn = a.conics + 6 * (size_t)c;
+        cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];
+        cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
+      }
+    }
+  };
+
+  for (uint32_t t = blockIdx.x; t < num_tiles; t += gridDim.x) {
+    // per view: exclusive offsets of the (item, wave) groups inside the tile
+    for (uint32_t v = 0; v < nv; ++v) {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t idx = t * TILE + j * THREADS + tid;
+        const uint32_t m = idx < n ? a.vis[idx] : 0u;
+        const uint64_t b = __ballot((m >> v) & 1u);
+        if (lane == 0) s_off[v][j * WAVES + w] = (uint32_t)__popcll(b);
+      }
+    }
+    __syncthreads();
+    for (uint32_t v = w; v < nv; v += WAVES) {
+      const uint32_t c = s_off[v][lane];
+      uint32_t incl = c;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= (uint32_t)o) incl += y;
+      }
+      s_off[v][lane] = incl - c;
+      if (lane == 0) s_tp[v] = a.tile_prefix[v * a.tp_stride + t];
+    }
+    __syncthreads();
+
+    uint32_t listed = 0;  // pairs in the list (uniform across the block)
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t base = j * THREADS;
+      const uint32_t mine = t * TILE + base + tid < n ? a.vis[t * TILE + base + tid] : 0u;
+      uint32_t total;
+      uint32_t slot = block_exclusive_scan<THREADS>((uint32_t)__popc(mine), s_tmp, total);
+      if (listed + total > CAP) {  // project what is listed, then start over
+        project_pairs(t, listed);
+        __syncthreads();
+        listed = 0;
+      }
+      slot += listed;
+      for (uint32_t v = 0; v < nv; ++v) {
+        const bool vis = (mine >> v) & 1u;
+        const uint64_t b = __ballot(vis);
+        if (vis) {
+          s_item[slot] = ((base + tid) << 5) | v;
+          s_entry[slot] = s_tp[v] + s_off[v][j * WAVES + w] + lanes_below(b);
+          ++slot;
         }
       }
+      listed += total;
       __syncthreads();
     }
+    project_pairs(t, listed);
+    __syncthreads();
   }
 }
 

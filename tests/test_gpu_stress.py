@@ -20,7 +20,8 @@ N, W, H, VIEWS = 32_000_000, 3840, 2160, 8
 
 def test_32m_tight_clusters_4k_8_views():
     t0 = time.time()
-    cfg = config(max_splat_count=N, max_views=VIEWS, sort_tile_keys=8192)
+    # dense clusters at 4K: up to ~4 screen tiles per splat
+    cfg = config(max_splat_count=N, max_views=VIEWS, sort_tile_keys=8192, tile_pairs_per_splat=8)
     r = sa.Renderer(device(), cfg)
     scene = sa.Scene(r, N)
     centers = np.empty((N, 3), np.float32)
