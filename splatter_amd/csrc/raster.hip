@@ -54,9 +54,6 @@ __device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f),
 
 }  // namespace
 
-// Views are visited one after another by the whole grid (each view's tiles
-// from its own counter), so the random reads of the draw order's pixel boxes
-// stay within one view's arrays, which the device's last-level cache holds.
 template <int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   constexpr int TILE = THREADS * ITEMS;
@@ -71,92 +68,90 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   __shared__ uint64_t s_prefix;
 
   const uint32_t tid = threadIdx.x;
-  for (uint32_t v = 0; v < (uint32_t)a.views; ++v) {
-    uint32_t* counters = a.counters + (size_t)v * kCounterWords;
-    const uint32_t* order = a.order ? a.order + v * a.order_stride : nullptr;
-    const uint2* boxes = a.boxes + v * a.box_stride;
-    uint32_t* pair_keys = a.pair_keys + v * a.pair_stride;
-    uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
-    uint64_t* status = a.status + v * a.status_stride;
-    for (int i = tid; i < a.pair_passes * 256; i += THREADS) s_hist[i] = 0u;
-    const uint32_t n = counters[kVisible];
-    const uint32_t num_tiles = (n + TILE - 1) / TILE;
-    if (num_tiles == 0 && blockIdx.x == 0 && tid == 0) {
-      counters[kPairs] = 0u;
-      counters[kPairOverflow] = 0u;
-    }
-    const uint32_t tiles_x = a.uniforms[v].tiles_x;
+  const uint32_t v = blockIdx.y;
+  uint32_t* counters = a.counters + (size_t)v * kCounterWords;
+  const uint32_t* order = a.order ? a.order + v * a.order_stride : nullptr;
+  const uint2* boxes = a.boxes + v * a.box_stride;
+  uint32_t* pair_keys = a.pair_keys + v * a.pair_stride;
+  uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
+  uint64_t* status = a.status + v * a.status_stride;
+  for (int i = tid; i < a.pair_passes * 256; i += THREADS) s_hist[i] = 0u;
+  const uint32_t n = counters[kVisible];
+  const uint32_t num_tiles = (n + TILE - 1) / TILE;
+  if (num_tiles == 0 && blockIdx.x == 0 && tid == 0) {
+    counters[kPairs] = 0u;
+    counters[kPairOverflow] = 0u;
+  }
+  const uint32_t tiles_x = a.uniforms[v].tiles_x;
 
-    while (true) {
-      if (tid == 0) s_tile = atomicAdd(&counters[kCtrBin], 1u);
-      __syncthreads();
-      const uint32_t t = s_tile;
-      if (t >= num_tiles) break;
-      uint32_t cnt[ITEMS], sum = 0;
-  #pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t slot = tid * ITEMS + k;
-        const uint32_t r = t * TILE + slot;
-        cnt[k] = 0;
-        if (r < n) {
-          const uint32_t c = order ? order[r] : r;
-          const uint2 q = boxes[c];
-          const uint32_t lo = q.x, hi = q.y;
-          const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;
-          const uint32_t x1 = (hi & 0xFFFFu) / kTileSize, y1 = (hi >> 16) / kTileSize;
-          if (lo != 0xFFFFFFFFu && x1 >= x0 && y1 >= y0) {
-            cnt[k] = (x1 - x0 + 1) * (y1 - y0 + 1);
-            s_org[slot] = x0 | (y0 << 16);
-            s_w[slot] = x1 - x0 + 1;
-          }
-          s_rec[slot] = c;
+  while (true) {
+    if (tid == 0) s_tile = atomicAdd(&counters[kCtrBin], 1u);
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= num_tiles) break;
+    uint32_t cnt[ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t slot = tid * ITEMS + k;
+      const uint32_t r = t * TILE + slot;
+      cnt[k] = 0;
+      if (r < n) {
+        const uint32_t c = order ? order[r] : r;
+        const uint2 q = boxes[c];
+        const uint32_t lo = q.x, hi = q.y;
+        const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;
+        const uint32_t x1 = (hi & 0xFFFFu) / kTileSize, y1 = (hi >> 16) / kTileSize;
+        if (lo != 0xFFFFFFFFu && x1 >= x0 && y1 >= y0) {
+          cnt[k] = (x1 - x0 + 1) * (y1 - y0 + 1);
+          s_org[slot] = x0 | (y0 << 16);
+          s_w[slot] = x1 - x0 + 1;
         }
-        sum += cnt[k];
+        s_rec[slot] = c;
       }
-      uint32_t total;
-      uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
-  #pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        s_off[tid * ITEMS + k] = excl;
-        excl += cnt[k];
-      }
-      if (tid < kWave) {
-        const uint64_t prefix = wave_lookback64(status, t, (uint64_t)total);
-        if (tid == 0) {
-          s_off[TILE] = total;
-          s_prefix = prefix;
-        }
-        if (tid == 0 && t == num_tiles - 1) {
-          const uint64_t k_all = prefix + total;
-          counters[kPairs] = (uint32_t)(k_all < a.pair_cap ? k_all : a.pair_cap);
-          counters[kPairOverflow] = k_all > a.pair_cap ? 1u : 0u;
-        }
-      }
-      __syncthreads();
-      const uint64_t prefix = s_prefix;
-      for (uint32_t s = tid; s < total; s += THREADS) {
-        uint32_t lo = 0, hi = TILE - 1;
-        while (lo < hi) {
-          const uint32_t mid = (lo + hi + 1) >> 1;
-          if (s_off[mid] <= s) lo = mid; else hi = mid - 1;
-        }
-        const uint64_t pos = prefix + s;
-        if (pos < a.pair_cap) {
-          const uint32_t k = s - s_off[lo];
-          const uint32_t w = s_w[lo], org = s_org[lo];
-          const uint32_t tile = ((org >> 16) + k / w) * tiles_x + (org & 0xFFFFu) + k % w;
-          pair_keys[pos] = tile;
-          pair_vals[pos] = s_rec[lo];
-          for (int q = 0; q < a.pair_passes; ++q) atomicAdd(&s_hist[q * 256 + ((tile >> (8 * q)) & 255u)], 1u);
-        }
-      }
-      __syncthreads();
+      sum += cnt[k];
     }
-    for (int i = tid; i < a.pair_passes * 256; i += THREADS)
-      if (s_hist[i]) atomicAdd(&a.hist[v * a.hist_stride + i], s_hist[i]);
-    finish_persistent(&counters[kCtrBin], &counters[kDone + kCtrBin]);
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      s_off[tid * ITEMS + k] = excl;
+      excl += cnt[k];
+    }
+    if (tid < kWave) {
+      const uint64_t prefix = wave_lookback64(status, t, (uint64_t)total);
+      if (tid == 0) {
+        s_off[TILE] = total;
+        s_prefix = prefix;
+      }
+      if (tid == 0 && t == num_tiles - 1) {
+        const uint64_t k_all = prefix + total;
+        counters[kPairs] = (uint32_t)(k_all < a.pair_cap ? k_all : a.pair_cap);
+        counters[kPairOverflow] = k_all > a.pair_cap ? 1u : 0u;
+      }
+    }
+    __syncthreads();
+    const uint64_t prefix = s_prefix;
+    for (uint32_t s = tid; s < total; s += THREADS) {
+      uint32_t lo = 0, hi = TILE - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_off[mid] <= s) lo = mid; else hi = mid - 1;
+      }
+      const uint64_t pos = prefix + s;
+      if (pos < a.pair_cap) {
+        const uint32_t k = s - s_off[lo];
+        const uint32_t w = s_w[lo], org = s_org[lo];
+        const uint32_t tile = ((org >> 16) + k / w) * tiles_x + (org & 0xFFFFu) + k % w;
+        pair_keys[pos] = tile;
+        pair_vals[pos] = s_rec[lo];
+        for (int q = 0; q < a.pair_passes; ++q) atomicAdd(&s_hist[q * 256 + ((tile >> (8 * q)) & 255u)], 1u);
+      }
+    }
     __syncthreads();
   }
+  for (int i = tid; i < a.pair_passes * 256; i += THREADS)
+    if (s_hist[i]) atomicAdd(&a.hist[v * a.hist_stride + i], s_hist[i]);
+  finish_persistent(&counters[kCtrBin], &counters[kDone + kCtrBin]);
 }
 
 // Run boundaries of the tile-sorted pair keys, four keys per thread.
@@ -383,7 +378,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
 }
 
 void launch_bin(const BinLaunch& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_bin<kBinThreads, kBinItems>), dim3(a.grid_x), dim3(kBinThreads), 0, s, a);
+  hipLaunchKernelGGL((k_bin<kBinThreads, kBinItems>), dim3(a.grid_x, a.views), dim3(kBinThreads), 0, s, a);
 }
 
 void launch_ranges(const RangesLaunch& a, hipStream_t s) {

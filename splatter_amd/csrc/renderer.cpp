@@ -260,10 +260,10 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   const int Q = b.pair_passes;
   const uint32_t cus = (uint32_t)r->dev->cu_count;
   const uint32_t per_view = std::max<uint32_t>(1, cus * 4 / nv);
-  // radix-sort tiles: two 512-lane or four 256-lane blocks per CU, the whole
-  // grid on one view at a time
+  // radix-sort tiles: two 512-lane or four 256-lane blocks per CU, shared
+  // by the views (which sort side by side, so their look-back chains overlap)
   const uint32_t sort_tile = (uint32_t)r->sort_threads * kSortItems;
-  const uint32_t sort_blocks = cus * 4 * kSortThreads / r->sort_threads;
+  const uint32_t sort_per_view = std::max<uint32_t>(1, cus * 4 * kSortThreads / r->sort_threads / nv);
   const size_t cap = r->cap;
   const size_t sort_stride = r->sort_words * kMaxDepthPasses;
   const size_t pair_stride_status = r->pair_words * kMaxPairPasses;
@@ -324,7 +324,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a_stride = sort_stride;
     }
     a.threads = r->sort_threads;
-    a.grid_x = clampg((uint64_t)div_up(n, sort_tile) * nv, sort_blocks);
+    a.grid_x = clampg(div_up(n, sort_tile), sort_per_view);
     a.views = (int)nv;
     launch_sort_pass(bits, a, q);
   }
@@ -371,7 +371,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.pair_passes = Q;
     a.status = r->bin_status; a.status_stride = r->bin_words;
     a.counters = r->counters;
-    a.grid_x = clampg(div_up(n, kBinThreads * kBinItems), cus * 4);  // the whole grid on one view at a time
+    a.grid_x = clampg(div_up(n, kBinThreads * kBinItems), per_view);
     a.views = (int)nv;
     launch_bin(a, q);
   }
@@ -400,7 +400,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a_stride = pair_stride_status;
     }
     a.threads = r->sort_threads;
-    a.grid_x = clampg((uint64_t)div_up(r->pair_cap, sort_tile) * nv, sort_blocks);
+    a.grid_x = clampg(div_up(r->pair_cap, sort_tile), sort_per_view);
     a.views = (int)nv;
     launch_sort_pass(8, a, q);
   }

@@ -13,9 +13,8 @@
 //    workgroup each, shaders.wgsl:406-420).
 //  - the global digit offsets come from the upsweep histogram and are
 //    scanned by every block on entry (radixSortB's serial scan disappears).
-//  - the views of a batch are sorted one after another by the whole grid,
-//    each with its own tile counter and status words, so the scatter's
-//    partial cache lines of one view meet in the last-level cache.
+//  - the views of a batch sort side by side (view = blockIdx.y), each with
+//    its own tile counter and status words.
 //  - decoupled look-back per digit with agent-scope status words
 //    (aggregate / inclusive flags + 30-bit count, as shaders.wgsl:438-457).
 //  - keys, then values, are reordered through LDS so that global stores of
@@ -42,195 +41,193 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & (kWave - 1), w = tid / kWave;
-  for (uint32_t v = 0; v < (uint32_t)a.views; ++v) {
-    if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
-    if (a.clear_b) clear_words(a.clear_b + v * a.clear_b_stride, a.clear_b_words);
-    uint32_t* counters = a.counters + (size_t)v * kCounterWords;
-    const uint32_t* keys_in = a.keys_in + v * a.entry_stride;
-    const uint32_t* vals_in = a.vals_in ? a.vals_in + v * a.entry_stride : nullptr;
-    uint32_t* keys_out = a.keys_out + v * a.entry_stride;
-    uint32_t* vals_out = a.vals_out + v * a.entry_stride;
-    const uint32_t* hist = a.hist + v * a.hist_stride;
-    uint32_t* status = a.status + v * a.status_stride;
-    uint32_t* tile_counter = counters + a.tile_slot;
+  const uint32_t v = blockIdx.y;
+  if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
+  if (a.clear_b) clear_words(a.clear_b + v * a.clear_b_stride, a.clear_b_words);
+  uint32_t* counters = a.counters + (size_t)v * kCounterWords;
+  const uint32_t* keys_in = a.keys_in + v * a.entry_stride;
+  const uint32_t* vals_in = a.vals_in ? a.vals_in + v * a.entry_stride : nullptr;
+  uint32_t* keys_out = a.keys_out + v * a.entry_stride;
+  uint32_t* vals_out = a.vals_out + v * a.entry_stride;
+  const uint32_t* hist = a.hist + v * a.hist_stride;
+  uint32_t* status = a.status + v * a.status_stride;
+  uint32_t* tile_counter = counters + a.tile_slot;
 
-    uint32_t n = counters[a.count_slot];
-    if (n > a.count_cap) n = a.count_cap;
-    const uint32_t num_tiles = (n + TILE - 1) / TILE;
-    const int shift = a.shift;
+  uint32_t n = counters[a.count_slot];
+  if (n > a.count_cap) n = a.count_cap;
+  const uint32_t num_tiles = (n + TILE - 1) / TILE;
+  const int shift = a.shift;
 
-    {  // global exclusive digit starts
-      uint32_t hv[DPT], sum = 0;
+  {  // global exclusive digit starts
+    uint32_t v[DPT], sum = 0;
 #pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const uint32_t d = tid * DPT + k;
-        hv[k] = d < (uint32_t)RADIX ? hist[d] : 0u;
-        sum += hv[k];
-      }
-      uint32_t total;
-      uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+    for (int k = 0; k < DPT; ++k) {
+      const uint32_t d = tid * DPT + k;
+      v[k] = d < (uint32_t)RADIX ? hist[d] : 0u;
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
 #pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const uint32_t d = tid * DPT + k;
-        if (d < (uint32_t)RADIX) s_digit_start[d] = excl;
-        excl += hv[k];
-      }
+    for (int k = 0; k < DPT; ++k) {
+      const uint32_t d = tid * DPT + k;
+      if (d < (uint32_t)RADIX) s_digit_start[d] = excl;
+      excl += v[k];
+    }
+  }
+
+  while (true) {
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < WAVES * RADIX; i += THREADS) s_wave_hist[i] = 0u;
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= num_tiles) break;
+    const uint32_t tile_base = t * TILE;
+    const uint32_t tile_n = (n - tile_base) < (uint32_t)TILE ? (n - tile_base) : (uint32_t)TILE;
+
+    // Register budget: keys die once they are in LDS, values are fetched
+    // while the look-back waits, and the scatter keeps digits packed (four
+    // 8-bit or two 16-bit to a register) instead of 32-bit positions.
+    uint32_t key[ITEMS], pos[ITEMS];
+    const uint32_t wave_base = tile_base + w * (kWave * ITEMS) + lane;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t idx = wave_base + j * kWave;
+      key[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;
     }
 
-    while (true) {
-      if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-      for (int i = tid; i < WAVES * RADIX; i += THREADS) s_wave_hist[i] = 0u;
-      __syncthreads();
-      const uint32_t t = s_tile;
-      if (t >= num_tiles) break;
-      const uint32_t tile_base = t * TILE;
-      const uint32_t tile_n = (n - tile_base) < (uint32_t)TILE ? (n - tile_base) : (uint32_t)TILE;
-
-      // Register budget: keys die once they are in LDS, values are fetched
-      // while the look-back waits, and the scatter keeps digits packed (four
-      // 8-bit or two 16-bit to a register) instead of 32-bit positions.
-      uint32_t key[ITEMS], pos[ITEMS];
-      const uint32_t wave_base = tile_base + w * (kWave * ITEMS) + lane;
+    // WLMS ranking, items in input order
+    volatile uint32_t* wh = s_wave_hist + w * RADIX;
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t idx = wave_base + j * kWave;
-        key[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;
+    for (int j = 0; j < ITEMS; ++j) {
+      const bool valid = (wave_base + j * kWave) < n;
+      const uint32_t d = (key[j] >> shift) & MASK;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < BITS; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
       }
+      const uint32_t below = lanes_below(peers);
+      const uint32_t cnt = (uint32_t)__popcll(peers);
+      uint32_t base = 0;
+      if (valid) base = wh[d];
+      __builtin_amdgcn_wave_barrier();
+      if (valid && below == 0) wh[d] = base + cnt;
+      __builtin_amdgcn_wave_barrier();
+      pos[j] = base + below;
+    }
+    __syncthreads();
 
-      // WLMS ranking, items in input order
-      volatile uint32_t* wh = s_wave_hist + w * RADIX;
+    // per digit: exclusive prefix over waves, tile count
+    uint32_t cnt_d[DPT], sum = 0;
 #pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const bool valid = (wave_base + j * kWave) < n;
+    for (int k = 0; k < DPT; ++k) {
+      const uint32_t d = tid * DPT + k;
+      uint32_t run = 0;
+      if (d < (uint32_t)RADIX) {
+#pragma unroll
+        for (int ww = 0; ww < WAVES; ++ww) {
+          const uint32_t x = s_wave_hist[ww * RADIX + d];
+          s_wave_hist[ww * RADIX + d] = run;
+          run += x;
+        }
+      }
+      cnt_d[k] = run;
+      sum += run;
+    }
+    // publish aggregates early so later tiles can move on
+    if (t > 0) {
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const uint32_t d = tid * DPT + k;
+        if (d < (uint32_t)RADIX) store_status(status + (size_t)t * RADIX + d, kFlagAggregate | cnt_d[k]);
+      }
+    }
+    uint32_t total;
+    const uint32_t excl0 = block_exclusive_scan<THREADS>(sum, s_tmp, total);
+    {
+      uint32_t excl = excl0;
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const uint32_t d = tid * DPT + k;
+        if (d < (uint32_t)RADIX) s_local[d] = excl;
+        excl += cnt_d[k];
+      }
+    }
+    __syncthreads();
+
+    // keys to their place in the tile, sorted by digit
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      if (wave_base + j * kWave < n) {
         const uint32_t d = (key[j] >> shift) & MASK;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < BITS; ++b) {
-          const bool bit = (d >> b) & 1u;
-          const uint64_t bal = __ballot(bit);
-          peers &= bit ? bal : ~bal;
-        }
-        const uint32_t below = lanes_below(peers);
-        const uint32_t cnt = (uint32_t)__popcll(peers);
-        uint32_t base = 0;
-        if (valid) base = wh[d];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && below == 0) wh[d] = base + cnt;
-        __builtin_amdgcn_wave_barrier();
-        pos[j] = base + below;
+        pos[j] += s_local[d] + s_wave_hist[w * RADIX + d];
+        if (pos[j] < (uint32_t)TILE) s_buf[pos[j]] = key[j];
       }
-      __syncthreads();
-
-      // per digit: exclusive prefix over waves, tile count
-      uint32_t cnt_d[DPT], sum = 0;
+    }
+    uint32_t val[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const uint32_t idx = wave_base + j * kWave;
+      val[j] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0u;
+    }
+    {
+      uint32_t excl = excl0;
 #pragma unroll
       for (int k = 0; k < DPT; ++k) {
         const uint32_t d = tid * DPT + k;
-        uint32_t run = 0;
         if (d < (uint32_t)RADIX) {
-#pragma unroll
-          for (int ww = 0; ww < WAVES; ++ww) {
-            const uint32_t x = s_wave_hist[ww * RADIX + d];
-            s_wave_hist[ww * RADIX + d] = run;
-            run += x;
-          }
+          const uint32_t prefix = t == 0 ? 0u : lookback_window<16>(status + d, t, RADIX);
+          store_status(status + (size_t)t * RADIX + d, kFlagInclusive | (prefix + cnt_d[k]));
+          s_base[d] = s_digit_start[d] + prefix - excl;
         }
-        cnt_d[k] = run;
-        sum += run;
+        excl += cnt_d[k];
       }
-      // publish aggregates early so later tiles can move on
-      if (t > 0) {
-#pragma unroll
-        for (int k = 0; k < DPT; ++k) {
-          const uint32_t d = tid * DPT + k;
-          if (d < (uint32_t)RADIX) store_status(status + (size_t)t * RADIX + d, kFlagAggregate | cnt_d[k]);
-        }
-      }
-      uint32_t total;
-      const uint32_t excl0 = block_exclusive_scan<THREADS>(sum, s_tmp, total);
-      {
-        uint32_t excl = excl0;
-#pragma unroll
-        for (int k = 0; k < DPT; ++k) {
-          const uint32_t d = tid * DPT + k;
-          if (d < (uint32_t)RADIX) s_local[d] = excl;
-          excl += cnt_d[k];
-        }
-      }
-      __syncthreads();
-
-      // keys to their place in the tile, sorted by digit
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        if (wave_base + j * kWave < n) {
-          const uint32_t d = (key[j] >> shift) & MASK;
-          pos[j] += s_local[d] + s_wave_hist[w * RADIX + d];
-          if (pos[j] < (uint32_t)TILE) s_buf[pos[j]] = key[j];
-        }
-      }
-      uint32_t val[ITEMS];
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const uint32_t idx = wave_base + j * kWave;
-        val[j] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0u;
-      }
-      {
-        uint32_t excl = excl0;
-#pragma unroll
-        for (int k = 0; k < DPT; ++k) {
-          const uint32_t d = tid * DPT + k;
-          if (d < (uint32_t)RADIX) {
-            const uint32_t prefix = t == 0 ? 0u : lookback_window<16>(status + d, t, RADIX);
-            store_status(status + (size_t)t * RADIX + d, kFlagInclusive | (prefix + cnt_d[k]));
-            s_base[d] = s_digit_start[d] + prefix - excl;
-          }
-          excl += cnt_d[k];
-        }
-      }
-      __syncthreads();
-
-      constexpr int PW = BITS <= 8 ? 8 : 16;  // bits per packed digit
-      constexpr int PER = 32 / PW;
-      uint32_t dig[(ITEMS + PER - 1) / PER];
-#pragma unroll
-      for (int k = 0; k < (ITEMS + PER - 1) / PER; ++k) dig[k] = 0u;
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = k * THREADS + tid;
-        if (i < tile_n) {
-          const uint32_t kk = s_buf[i];
-          const uint32_t d = (kk >> shift) & MASK;
-          dig[k / PER] |= d << (PW * (k % PER));
-          const uint32_t o = s_base[d] + i;
-          // a position past n means the histogram and the keys disagree:
-          // report it instead of storing out of bounds
-          if (o < n) keys_out[o] = kk;
-          else counters[kScatterError] = 1u;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j)
-        if (wave_base + j * kWave < n && pos[j] < (uint32_t)TILE) s_buf[pos[j]] = val[j];
-      __syncthreads();
-#pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = k * THREADS + tid;
-        if (i < tile_n) {
-          const uint32_t d = (dig[k / PER] >> (PW * (k % PER))) & ((1u << PW) - 1u);
-          const uint32_t o = s_base[d] + i;
-          if (o < n) vals_out[o] = s_buf[i];
-        }
-      }
-      __syncthreads();
     }
-    finish_persistent(tile_counter, counters + kDone + a.tile_slot);
+    __syncthreads();
+
+    constexpr int PW = BITS <= 8 ? 8 : 16;  // bits per packed digit
+    constexpr int PER = 32 / PW;
+    uint32_t dig[(ITEMS + PER - 1) / PER];
+#pragma unroll
+    for (int k = 0; k < (ITEMS + PER - 1) / PER; ++k) dig[k] = 0u;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t i = k * THREADS + tid;
+      if (i < tile_n) {
+        const uint32_t kk = s_buf[i];
+        const uint32_t d = (kk >> shift) & MASK;
+        dig[k / PER] |= d << (PW * (k % PER));
+        const uint32_t o = s_base[d] + i;
+        // a position past n means the histogram and the keys disagree:
+        // report it instead of storing out of bounds
+        if (o < n) keys_out[o] = kk;
+        else counters[kScatterError] = 1u;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+      if (wave_base + j * kWave < n && pos[j] < (uint32_t)TILE) s_buf[pos[j]] = val[j];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t i = k * THREADS + tid;
+      if (i < tile_n) {
+        const uint32_t d = (dig[k / PER] >> (PW * (k % PER))) & ((1u << PW) - 1u);
+        const uint32_t o = s_base[d] + i;
+        if (o < n) vals_out[o] = s_buf[i];
+      }
+    }
     __syncthreads();
   }
+  finish_persistent(tile_counter, counters + kDone + a.tile_slot);
 }
 
 void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s) {
-  const dim3 grid(a.grid_x);
+  const dim3 grid(a.grid_x, a.views);
   if (a.threads == kSortThreadsWide) {
     if (bits == 10)
       hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreadsWide, kSortItems>), grid, dim3(kSortThreadsWide), 0, s, a);
