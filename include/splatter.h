@@ -92,6 +92,8 @@ typedef struct spl_configuration {
     uint32_t max_views;                   /* views per batched launch (1..=32); workspace scales with it */
     int32_t use_hip_graphs;               /* replay each batch shape's launch sequence from a hipGraph */
     uint32_t sort_tile_keys;              /* keys per radix-sort tile: 4096 (256 lanes) or 8192 (512 lanes) */
+    uint32_t raster_walk;                 /* 0: every lane tests each listed entry; 1: each lane walks the
+                                             entries covering its pixel (same image) */
 } spl_configuration;
 
 const char* spl_last_error(void);

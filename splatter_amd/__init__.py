@@ -71,6 +71,7 @@ class Configuration:
     max_views: int = 1
     use_hip_graphs: bool = True
     sort_tile_keys: int = 4096
+    raster_walk: int = 0
 
     def to_c(self) -> SplConfiguration:
         c = SplConfiguration()
@@ -90,6 +91,7 @@ class Configuration:
         c.max_views = int(self.max_views)
         c.use_hip_graphs = int(bool(self.use_hip_graphs))
         c.sort_tile_keys = int(self.sort_tile_keys)
+        c.raster_walk = int(self.raster_walk)
         return c
 
 

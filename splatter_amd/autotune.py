@@ -1,7 +1,8 @@
 """Autotuner for the batched form: picks the radix digit width, the radix-sort
-tile (keys per tile, i.e. the sort block's lane count) and the batch width
-(views per launch sequence) for one scene, resolution and pose set by timing
-every candidate on the device, and can keep the answer in a JSON cache.
+tile (keys per tile, i.e. the sort block's lane count), the rasteriser's walk
+(entry-major or per-pixel) and the batch width (views per launch sequence)
+for one scene, resolution and pose set by timing every candidate on the
+device, and can keep the answer in a JSON cache.
 
 The candidates all render identical images and sort orders (the tests check
 each option against the oracle), so the tuner is free to pick on speed alone.
@@ -18,6 +19,7 @@ import numpy as np
 
 RADIX_BITS = (8, 10)
 SORT_TILE_KEYS = (4096, 8192)
+RASTER_WALK = (0, 1)
 BATCH_VIEWS = (8, 16, 32)
 
 
@@ -26,9 +28,10 @@ class Candidate:
     radix_bits_per_digit: int
     sort_tile_keys: int
     max_views: int
+    raster_walk: int = 0
 
     def key(self) -> str:
-        return f"r{self.radix_bits_per_digit}-t{self.sort_tile_keys}-v{self.max_views}"
+        return f"r{self.radix_bits_per_digit}-t{self.sort_tile_keys}-w{self.raster_walk}-v{self.max_views}"
 
 
 @dataclass
@@ -41,6 +44,7 @@ class TuneResult:
         config.radix_bits_per_digit = self.best.radix_bits_per_digit
         config.sort_tile_keys = self.best.sort_tile_keys
         config.max_views = self.best.max_views
+        config.raster_walk = self.best.raster_walk
 
     def to_json(self) -> dict:
         return {"best": self.best.__dict__, "frames_per_s": self.frames_per_s, "table": self.table}
@@ -50,8 +54,10 @@ class TuneResult:
         return cls(Candidate(**d["best"]), float(d["frames_per_s"]), [tuple(x) for x in d.get("table", [])])
 
 
-def candidates(radix_bits=RADIX_BITS, sort_tile_keys=SORT_TILE_KEYS, batch_views=BATCH_VIEWS):
-    return [Candidate(r, t, v) for r, t, v in itertools.product(radix_bits, sort_tile_keys, batch_views)]
+def candidates(radix_bits=RADIX_BITS, sort_tile_keys=SORT_TILE_KEYS, batch_views=BATCH_VIEWS,
+               raster_walk=RASTER_WALK):
+    return [Candidate(r, t, v, w) for r, t, v, w in
+            itertools.product(radix_bits, sort_tile_keys, batch_views, raster_walk)]
 
 
 def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand: Candidate,
@@ -61,6 +67,7 @@ def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand
     cfg.radix_bits_per_digit = cand.radix_bits_per_digit
     cfg.sort_tile_keys = cand.sort_tile_keys
     cfg.max_views = cand.max_views
+    cfg.raster_walk = cand.raster_walk
     r = sa.Renderer(device, cfg)
     try:
         scene = scene_loader(r)

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
 // then keeps, in order, only the entries whose pixel box meets its quadrant
 // (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
 // stops blending, and the block stops when all four have.
-template <bool UNALIGNED, bool F32>
+template <bool UNALIGNED, bool F32, bool WALK>
 __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   constexpr int B = 256;
   __shared__ float4 s_m[2][B];     // i00 i01 i10 i11
@@ -316,45 +316,86 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
         n += (uint32_t)__popcll(bm);
       }
       __builtin_amdgcn_wave_barrier();
-      // four list slots per step: their LDS reads are issued together
-      for (uint32_t i = 0; i < n; i += 4) {
-        const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][i]);
-        float4 m[4], t[4], col[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const uint32_t j = (js >> (8 * k)) & 255u;  // slots past n hold stale, in-range indices
-          m[k] = s_m[buf][j];
-          t[k] = s_t[buf][j];
-          col[k] = s_c[buf][j];
+      // one fragment of entry slot j at this lane's pixel: coverage, alpha
+      // (shaders.wgsl:585-589) and the blend; identical arithmetic on both
+      // walks below, so they give identical pixels
+      auto covers = [&](const float4& m, const float4& t, float& u, float& w2) -> bool {
+        const float dx = qx - t.x, dy = qy - t.y;
+        u = fmaf(m.x, dx, m.y * dy);  // S * texture coordinate
+        w2 = fmaf(m.z, dx, m.w * dy);
+        const float h = t.z;
+        return unaligned ? fmaxf(fabsf(u), fabsf(w2)) < h : fmaxf(fabsf(dx), fabsf(dy)) < h;
+      };
+      auto shade = [&](float u, float w2, float opacity, const float4& col) {
+        const float alpha = opacity * spl_exp2(-fmaf(u, u, w2 * w2));
+        if (alpha >= 1.0f / 255.0f) {
+          const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
+          if constexpr (!f32) {
+            // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
+            // 255) and rint(T * (1 - a)): the ideal 8-bit blender
+            const float sa = clamp01(alpha);
+            c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
+            c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
+            c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
+            tr = rintf(fmaf(-sa, tr, tr));
+          } else {
+            const float da = tr;
+            c0 = fmaf(sr, da, c0);
+            c1 = fmaf(sg, da, c1);
+            c2 = fmaf(sb, da, c2);
+            tr = fmaf(-alpha, da, da);
+          }
         }
+      };
+      if constexpr (!WALK) {
+        // entry-major: the wave steps through its list, every lane testing
+        // its pixel; four list slots per step so their LDS reads go together
+        for (uint32_t i = 0; i < n; i += 4) {
+          const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][i]);
+          float4 m[4], t[4], col[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          if (i + k >= n) break;
-          const float dx = qx - t[k].x, dy = qy - t[k].y;
-          const float u = fmaf(m[k].x, dx, m[k].y * dy);  // S * texture coordinate
-          const float w2 = fmaf(m[k].z, dx, m[k].w * dy);
-          const float h = t[k].z;
-          const bool in = unaligned ? fmaxf(fabsf(u), fabsf(w2)) < h : fmaxf(fabsf(dx), fabsf(dy)) < h;
-          if (in && tr != 0.0f) {
-            const float alpha = t[k].w * spl_exp2(-fmaf(u, u, w2 * w2));
-            if (alpha >= 1.0f / 255.0f) {
-              const float sr = col[k].x * alpha, sg = col[k].y * alpha, sb = col[k].z * alpha;
-              if constexpr (!f32) {
-                // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
-                // 255) and rint(T * (1 - a)): the ideal 8-bit blender
-                const float sa = clamp01(alpha);
-                c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
-                c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
-                c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
-                tr = rintf(fmaf(-sa, tr, tr));
-              } else {
-                const float da = tr;
-                c0 = fmaf(sr, da, c0);
-                c1 = fmaf(sg, da, c1);
-                c2 = fmaf(sb, da, c2);
-                tr = fmaf(-alpha, da, da);
-              }
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t j = (js >> (8 * k)) & 255u;  // slots past n hold stale, in-range indices
+            m[k] = s_m[buf][j];
+            t[k] = s_t[buf][j];
+            col[k] = s_c[buf][j];
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            if (i + k >= n) break;
+            float u, w2;
+            if (covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k]);
+          }
+        }
+      } else {
+        // pixel-major: for 64 list slots at a time every lane first marks the
+        // entries that cover its pixel, then walks only those, in order, so
+        // lanes are not held up by entries that cover few of the 64 pixels
+        for (uint32_t c = 0; c < n; c += 64) {
+          const uint32_t cnt = min(64u, n - c);
+          uint32_t lo = 0, hi = 0;
+          for (uint32_t k = 0; k < cnt; k += 4) {
+            const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][c + k]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const uint32_t j = (js >> (8 * q)) & 255u;
+              float u, w2;
+              const uint32_t in = covers(s_m[buf][j], s_t[buf][j], u, w2) && k + q < cnt ? 1u : 0u;
+              if (k < 32) lo |= in << (k + q);
+              else hi |= in << (k + q - 32);
             }
+          }
+          if (tr == 0.0f) lo = hi = 0u;
+          while (lo | hi) {
+            uint32_t k;
+            if (lo) { k = (uint32_t)__builtin_ctz(lo); lo &= lo - 1u; }
+            else { k = 32u + (uint32_t)__builtin_ctz(hi); hi &= hi - 1u; }
+            const uint32_t j = s_list[w][c + k];
+            const float4 t = s_t[buf][j];
+            float u, w2;
+            (void)covers(s_m[buf][j], t, u, w2);
+            shade(u, w2, t.w, s_c[buf][j]);
+            if (tr == 0.0f) lo = hi = 0u;
           }
         }
       }
@@ -387,12 +428,17 @@ void launch_ranges(const RangesLaunch& a, hipStream_t s) {
 
 void launch_raster(const RasterLaunch& a, hipStream_t s) {
   const dim3 grid(a.grid_x, a.views);
-  if (a.use_unaligned_rectangles) {
-    if (a.blend_f32) hipLaunchKernelGGL((k_raster<true, true>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_raster<true, false>), grid, dim3(256), 0, s, a);
-  } else {
-    if (a.blend_f32) hipLaunchKernelGGL((k_raster<false, true>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_raster<false, false>), grid, dim3(256), 0, s, a);
+  const dim3 block(256);
+  const int mode = (a.use_unaligned_rectangles ? 4 : 0) | (a.blend_f32 ? 2 : 0) | (a.walk ? 1 : 0);
+  switch (mode) {
+    case 0: hipLaunchKernelGGL((k_raster<false, false, false>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_raster<false, false, true>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_raster<false, true, false>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_raster<false, true, true>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_raster<true, false, false>), grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_raster<true, false, true>), grid, block, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_raster<true, true, false>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((k_raster<true, true, true>), grid, block, 0, s, a); break;
   }
 }
 

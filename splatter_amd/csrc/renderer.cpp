@@ -434,6 +434,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.counters = r->counters;
     a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
     a.blend_f32 = cfg.blend_mode == SPL_BLEND_F32 ? 1 : 0;
+    a.walk = cfg.raster_walk ? 1 : 0;
     a.bgra = cfg.frame_format == SPL_FRAME_FORMAT_BGRA8_UNORM ? 1 : 0;
     a.tile_work = b.tile_stats ? r->tile_work : nullptr;
     a.grid_x = (int)(((b.max_tiles + 7) / 8) * 8);
@@ -580,6 +581,7 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
     return fail(SPL_ERROR_INVALID_ARGUMENT, "max_views must be in 1..=32");
   if (cfg->sort_tile_keys != 0 && cfg->sort_tile_keys != 4096 && cfg->sort_tile_keys != 8192)
     return fail(SPL_ERROR_UNSUPPORTED, "sort_tile_keys must be 4096 or 8192");
+  if (cfg->raster_walk > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_walk must be 0 or 1");
   HIP_TRY(hipSetDevice(dev->ordinal));
   auto* r = new (std::nothrow) spl_renderer();
   if (!r) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: host allocation");

@@ -96,21 +96,30 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
       key[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;
     }
 
-    // WLMS ranking, items in input order
-    volatile uint32_t* wh = s_wave_hist + w * RADIX;
+    // WLMS ranking, items in input order.  The wave's digit counters are
+    // read and bumped through an LDS-qualified volatile pointer: the peers'
+    // reads must see the leader's store of the previous item, and a generic
+    // pointer would turn them into flat accesses that wait on global loads.
+    volatile __attribute__((address_space(3))) uint32_t* wh =
+        (volatile __attribute__((address_space(3))) uint32_t*)(s_wave_hist + w * RADIX);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const bool valid = (wave_base + j * kWave) < n;
       const uint32_t d = (key[j] >> shift) & MASK;
-      uint64_t peers = __ballot(valid);
+      const uint64_t vm = __ballot(valid);
+      uint32_t plo = (uint32_t)vm, phi = (uint32_t)(vm >> 32);
 #pragma unroll
       for (int b = 0; b < BITS; ++b) {
-        const bool bit = (d >> b) & 1u;
+        // lanes whose bit b equals mine: the ballot, or its complement
+        const uint32_t bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
+        const uint32_t flip = bit - 1u;  // 0 when set, all ones when clear
+        plo &= (uint32_t)bal ^ flip;
+        phi &= (uint32_t)(bal >> 32) ^ flip;
       }
+      const uint64_t peers = ((uint64_t)phi << 32) | plo;
       const uint32_t below = lanes_below(peers);
-      const uint32_t cnt = (uint32_t)__popcll(peers);
+      const uint32_t cnt = (uint32_t)__popc(plo) + (uint32_t)__popc(phi);
       uint32_t base = 0;
       if (valid) base = wh[d];
       __builtin_amdgcn_wave_barrier();

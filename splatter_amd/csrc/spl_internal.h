@@ -157,6 +157,7 @@ struct RasterLaunch {
   uint32_t* counters;
   int use_unaligned_rectangles;
   int blend_f32;
+  int walk;                  // 1: per-pixel walk of covering entries, 0: entry-major
   int bgra;
   uint2* tile_work;          // optional, view 0: per tile (list length, entries consumed)
   int grid_x, views;

@@ -111,12 +111,19 @@ def test_depth_sorting_none_draws_in_index_order():
     check_case(case, conics=False)
 
 
-@pytest.mark.parametrize("bits,tile", [(10, 4096), (8, 8192), (10, 8192)])
-def test_radix_digit_width_and_sort_tile(bits, tile):
-    # every option the autotuner may pick sorts to the same bit-exact order
-    say(f"radix bits {bits}, sort tile {tile}")
-    case = Case(Workload("gaussian", 200_000, seed=5), config(radix_bits_per_digit=bits, sort_tile_keys=tile),
-                800, 600, sphere_pose(3))
+@pytest.mark.parametrize("bits,tile,walk", [(10, 4096, 0), (8, 8192, 0), (10, 8192, 1), (8, 4096, 1)])
+def test_autotuner_options(bits, tile, walk):
+    # every option the autotuner may pick gives the same bit-exact order and image
+    say(f"radix bits {bits}, sort tile {tile}, raster walk {walk}")
+    case = Case(Workload("gaussian", 200_000, seed=5),
+                config(radix_bits_per_digit=bits, sort_tile_keys=tile, raster_walk=walk), 800, 600, sphere_pose(3))
+    check_case(case, conics=False)
+
+
+@pytest.mark.parametrize("unaligned,blend", [(True, sa.BlendMode.F32), (False, sa.BlendMode.UNORM8)])
+def test_raster_walk_other_modes(unaligned, blend):
+    case = Case(Workload("clustered", 300_000, seed=8),
+                config(raster_walk=1, use_unaligned_rectangles=unaligned, blend_mode=blend), 640, 480, sphere_pose(8))
     check_case(case, conics=False)
 
 
