@@ -23,31 +23,7 @@ namespace spl {
 
 namespace {
 
-// WGSL column-major 3x3: m[c][r].  (X Y)[c][r] = sum_k X[k][r] Y[c][k].
-__device__ __forceinline__ void mat3_mul(const double X[3][3], const double Y[3][3], double out[3][3]) {
-#pragma clang fp contract(fast)
-  double tmp[3][3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) tmp[c][r] = (X[0][r] * Y[c][0] + X[1][r] * Y[c][1]) + X[2][r] * Y[c][2];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) out[c][r] = tmp[c][r];
-}
-
-__device__ __forceinline__ void mat3_transpose(const double X[3][3], double out[3][3]) {
-  double tmp[3][3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) tmp[c][r] = X[r][c];
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-#pragma unroll
-    for (int r = 0; r < 3; ++r) out[c][r] = tmp[c][r];
-}
+// Matrices are WGSL column-major 3x3: m[c][r].
 
 __device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
 
@@ -77,33 +53,38 @@ __device__ __forceinline__ void contour_transform(const double scale[3], const d
 }
 
 // shaders.wgsl:125-169 projectedContourOfEllipsoid, given the transform.
-__device__ void contour(const double T0[3][3], const double pos[3], const Uniforms& u, double M[3][3]) {
+// With B = C^T T and lo = T^T (camera - position) the reference forms
+// A = (1 - |lo|^2) I + lo lo^T and M = B A B^T; here M = s B B^T + (B lo)(B lo)^T
+// with s = 1 - |lo|^2, the same matrix in fewer FP64 operations (its six
+// distinct entries only).
+__device__ void contour(const double T[3][3], const double pos[3], const Uniforms& u, double M[3][3]) {
 #pragma clang fp contract(fast)
-  double C[3][3], T[3][3], A[3][3], Ct[3][3], Tt[3][3];
+  double C[3][3], B[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      C[c][r] = (double)u.camera_matrix[4 * c + r];
-      T[c][r] = T0[c][r];
-    }
-  double ro[3], lo[3], ls[3];
+    for (int r = 0; r < 3; ++r) C[c][r] = (double)u.camera_matrix[4 * c + r];
+  double ro[3], lo[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) ro[i] = (double)u.camera_matrix[12 + i] - pos[i];
 #pragma unroll
   for (int c = 0; c < 3; ++c) lo[c] = (ro[0] * T[c][0] + ro[1] * T[c][1]) + ro[2] * T[c][2];
+  const double s = ((1.0 - lo[0] * lo[0]) - lo[1] * lo[1]) - lo[2] * lo[2];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) ls[i] = lo[i] * lo[i];
-  const double dg0 = (1.0 - ls[1]) - ls[2], dg1 = (1.0 - ls[0]) - ls[2], dg2 = (1.0 - ls[0]) - ls[1];
-  const double tr0 = lo[1] * lo[2], tr1 = lo[0] * lo[2], tr2 = lo[0] * lo[1];
-  A[0][0] = dg0; A[0][1] = tr2; A[0][2] = tr1;
-  A[1][0] = tr2; A[1][1] = dg1; A[1][2] = tr0;
-  A[2][0] = tr1; A[2][1] = tr0; A[2][2] = dg2;
-  mat3_transpose(C, Ct);
-  mat3_mul(Ct, T, T);
-  mat3_transpose(T, Tt);
-  mat3_mul(T, A, M);
-  mat3_mul(M, Tt, M);
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) B[c][r] = (C[r][0] * T[c][0] + C[r][1] * T[c][1]) + C[r][2] * T[c][2];
+  double bl[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) bl[r] = (B[0][r] * lo[0] + B[1][r] * lo[1]) + B[2][r] * lo[2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = c; r < 3; ++r) {
+      const double bb = (B[0][r] * B[0][c] + B[1][r] * B[1][c]) + B[2][r] * B[2][c];
+      M[c][r] = s * bb + bl[r] * bl[c];
+      M[r][c] = M[c][r];
+    }
 }
 
 // shaders.wgsl:101-104: the view-independent half of
@@ -116,35 +97,43 @@ __device__ __forceinline__ void covariance_transform(const double scale[3], cons
     for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] * scale[c];
 }
 
-// shaders.wgsl:99-118 projectedCovarianceOfEllipsoid, given the transform.
-__device__ void covariance(const double T[3][3], const double pos[3], const Uniforms& u, double cov[3][3]) {
+// shaders.wgsl:99-118 projectedCovarianceOfEllipsoid, given the transform:
+// cov = W^T W with W = T^T C J.  Only the top-left 2x2 of cov is used
+// (extractScaleOfCovariance, shaders.wgsl:240-246) and J's third column and
+// two of its entries are zero, so only W's first two columns and three
+// entries of cov are formed.
+__device__ void covariance2(const double T[3][3], const double pos[3], const Uniforms& u,
+                            double& c00, double& c11, double& c01) {
 #pragma clang fp contract(fast)
-  double C[3][3], J[3][3], Tt[3][3];
+  double C[3][3], G[3][3];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
     for (int r = 0; r < 3; ++r) C[c][r] = (double)u.camera_matrix[4 * c + r];
   const float* V = u.view_matrix;
-  double vp[4];
+  double vp[3];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-    vp[r] = (((double)V[r] * pos[0] + (double)V[4 + r] * pos[1]) + (double)V[8 + r] * pos[2]) +
-            (double)V[12 + r] * 1.0;
+  for (int r = 0; r < 3; ++r)
+    vp[r] = (((double)V[r] * pos[0] + (double)V[4 + r] * pos[1]) + (double)V[8 + r] * pos[2]) + (double)V[12 + r];
   const double iz = 1.0 / vp[2];
   double qx = vp[0] * iz, qy = vp[1] * iz;
   qx = qx < -1.0 ? -1.0 : (qx > 1.0 ? 1.0 : qx);
   qy = qy < -1.0 ? -1.0 : (qy > 1.0 ? 1.0 : qy);
-  vp[0] = qx * vp[2];
-  vp[1] = qy * vp[2];
-  J[0][0] = iz; J[0][1] = 0.0; J[0][2] = -vp[0] * (iz * iz);
-  J[1][0] = 0.0; J[1][1] = iz; J[1][2] = -vp[1] * (iz * iz);
-  J[2][0] = 0.0; J[2][1] = 0.0; J[2][2] = 0.0;
-  mat3_transpose(T, Tt);
-  mat3_mul(Tt, C, Tt);
-  mat3_mul(Tt, J, Tt);
-  double Ttt[3][3];
-  mat3_transpose(Tt, Ttt);
-  mat3_mul(Ttt, Tt, cov);
+  const double j02 = -(qx * vp[2]) * (iz * iz), j12 = -(qy * vp[2]) * (iz * iz);
+  // G = T^T C
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) G[c][r] = (T[r][0] * C[c][0] + T[r][1] * C[c][1]) + T[r][2] * C[c][2];
+  double w0[3], w1[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    w0[r] = G[0][r] * iz + G[2][r] * j02;
+    w1[r] = G[1][r] * iz + G[2][r] * j12;
+  }
+  c00 = (w0[0] * w0[0] + w0[1] * w0[1]) + w0[2] * w0[2];
+  c11 = (w1[0] * w1[0] + w1[1] * w1[1]) + w1[2] * w1[2];
+  c01 = (w0[0] * w1[0] + w0[1] * w1[1]) + w0[2] * w1[2];
 }
 
 // shaders.wgsl:248-294 (f32; same constants and evaluation order)
@@ -235,14 +224,17 @@ __device__ void pixel_box(float A00, float A10, float A01, float A11, float tx, 
 // views that do not see their splat even when a frame has one view, and the
 // lanes of one splat read its attributes from the same cache lines.
 template <int L>
-__global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_project(ProjectLaunch a) {
   constexpr int THREADS = kCullThreads, ITEMS = kCullItems, WAVES = THREADS / kWave, TILE = kCullTile;
-  constexpr int CAP = 4096;  // listed pairs per round (one group of 256 splats has at most 256 * kMaxViews)
-  static_assert(THREADS * kMaxViews <= 2 * CAP, "a group's pairs fit after one flush");
+  // Pairs are listed for up to 16 views at a time (a larger batch lists its
+  // views in halves), so one group of 256 splats lists at most CAP pairs and
+  // a pair packs into 16 bits: slot in the tile << 4 | view - first view.
+  constexpr int CAP = THREADS * 16;
+  static_assert(TILE <= 4096 && kMaxViews <= 32, "pair packing");
   __shared__ uint32_t s_off[kMaxViews][ITEMS * WAVES];
   __shared__ uint32_t s_tp[kMaxViews];
-  __shared__ uint32_t s_item[2 * CAP];  // slot in tile << 5 | view
-  __shared__ uint32_t s_entry[2 * CAP]; // entry c of that view
+  __shared__ uint16_t s_item[CAP];    // slot in tile << 4 | view - first view
+  __shared__ uint32_t s_entry[CAP];   // entry c of that view
   __shared__ uint32_t s_tmp[WAVES];
   const uint32_t nv = a.views;
   for (uint32_t v = 0; v < nv; ++v) {
@@ -256,12 +248,12 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
   constexpr int SH_F4 = (SH_FLOATS + 3) / 4;
   const double splat_scale = (double)a.uniforms[0].splat_scale;
 
-  // projects listed pairs [0, count) of tile t
-  auto project_pairs = [&](uint32_t t, uint32_t count) {
+  // projects listed pairs [0, count) of tile t, views from v0
+  auto project_pairs = [&](uint32_t t, uint32_t v0, uint32_t count) {
     for (uint32_t k = tid; k < count; k += THREADS) {
       const uint32_t it = s_item[k];
-      const uint32_t v = it & 31u;
-      const uint32_t i = t * TILE + (it >> 5);
+      const uint32_t v = v0 + (it & 15u);
+      const uint32_t i = t * TILE + (it >> 4);
       const uint32_t c = s_entry[k];
       const Uniforms& u = a.uniforms[v];
 
@@ -284,6 +276,10 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
       rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
       float rgb[3];
       sh_color<L>(rd, sh, rgb);
+      float4* rec = a.records + v * a.rec_stride + 3 * (size_t)c;
+      rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
+      // the colour's registers are free before the FP64 part starts
+      __builtin_amdgcn_sched_barrier(0);
 
       // shaders.wgsl:533-547
       const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
@@ -313,13 +309,13 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
         double s0, s1;
         if (a.use_covariance_for_scale) {
           // extractScaleOfCovariance (shaders.wgsl:240-246)
-          double Tv[3][3], cov[3][3];
+          double Tv[3][3], c00, c11, c01;
           covariance_transform(scale, rot, Tv);
-          covariance(Tv, pos, u, cov);
-          const double cd = cov[0][0] - cov[1][1];
-          const double bb = sqrt(cd * cd + (4.0 * cov[0][1]) * cov[0][1]);
-          s0 = sqrt(((cov[0][0] + cov[1][1]) + bb) * 0.5);
-          s1 = sqrt(((cov[0][0] + cov[1][1]) - bb) * 0.5);
+          covariance2(Tv, pos, u, c00, c11, c01);
+          const double cd = c00 - c11;
+          const double bb = sqrt(cd * cd + (4.0 * c01) * c01);
+          s0 = sqrt(((c00 + c11) + bb) * 0.5);
+          s1 = sqrt(((c00 + c11) - bb) * 0.5);
         } else {
           // extractScaleOfEllipse (shaders.wgsl:217-237)
           const double d = ((2.0 * M[0][1]) * jj) * kk;
@@ -335,8 +331,6 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
       }
       uint32_t box_lo, box_hi;
       pixel_box(A00, A10, A01, A11, tx, ty, u, a.use_unaligned_rectangles != 0, box_lo, box_hi);
-      float4* rec = a.records + v * a.rec_stride + 3 * (size_t)c;
-      rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
       rec[1] = make_float4(A00, A10, A01, A11);
       rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
       a.boxes[v * a.box_stride + c] = make_uint2(box_lo, box_hi);
@@ -373,32 +367,36 @@ __global__ __launch_bounds__(256) void k_project(ProjectLaunch a) {
     }
     __syncthreads();
 
-    uint32_t listed = 0;  // pairs in the list (uniform across the block)
-    for (int j = 0; j < ITEMS; ++j) {
-      const uint32_t base = j * THREADS;
-      const uint32_t mine = t * TILE + base + tid < n ? a.vis[t * TILE + base + tid] : 0u;
-      uint32_t total;
-      uint32_t slot = block_exclusive_scan<THREADS>((uint32_t)__popc(mine), s_tmp, total);
-      if (listed + total > CAP) {  // project what is listed, then start over
-        project_pairs(t, listed);
-        __syncthreads();
-        listed = 0;
-      }
-      slot += listed;
-      for (uint32_t v = 0; v < nv; ++v) {
-        const bool vis = (mine >> v) & 1u;
-        const uint64_t b = __ballot(vis);
-        if (vis) {
-          s_item[slot] = ((base + tid) << 5) | v;
-          s_entry[slot] = s_tp[v] + s_off[v][j * WAVES + w] + lanes_below(b);
-          ++slot;
+    for (uint32_t v0 = 0; v0 < nv; v0 += 16) {
+      const uint32_t nh = min(16u, nv - v0);
+      uint32_t listed = 0;  // pairs in the list (uniform across the block)
+      for (int j = 0; j < ITEMS; ++j) {
+        const uint32_t base = j * THREADS;
+        const uint32_t all = t * TILE + base + tid < n ? a.vis[t * TILE + base + tid] : 0u;
+        const uint32_t mine = (all >> v0) & ((1u << nh) - 1u);
+        uint32_t total;
+        uint32_t slot = block_exclusive_scan<THREADS>((uint32_t)__popc(mine), s_tmp, total);
+        if (listed + total > (uint32_t)CAP) {  // project what is listed, then start over
+          project_pairs(t, v0, listed);
+          __syncthreads();
+          listed = 0;
         }
+        slot += listed;
+        for (uint32_t h = 0; h < nh; ++h) {
+          const bool vis = (mine >> h) & 1u;
+          const uint64_t b = __ballot(vis);
+          if (vis) {
+            s_item[slot] = (uint16_t)(((base + tid) << 4) | h);
+            s_entry[slot] = s_tp[v0 + h] + s_off[v0 + h][j * WAVES + w] + lanes_below(b);
+            ++slot;
+          }
+        }
+        listed += total;
+        __syncthreads();
       }
-      listed += total;
+      project_pairs(t, v0, listed);
       __syncthreads();
     }
-    project_pairs(t, listed);
-    __syncthreads();
   }
 }
 

@@ -355,7 +355,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a_words = cull_used;
       a.clear_a_stride = r->cull_tiles;
     }
-    a.grid = clampg(div_up(n, kCullTile), cus * 4);
+    a.grid = clampg(div_up(n, kCullTile), cus * 3);  // three resident blocks per CU
     launch_project(a, q);
   }
   mark(4);
