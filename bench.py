@@ -103,14 +103,9 @@ SERVING_SCENES = [300_000, 600_000, 1_000_000, 1_500_000, 2_000_000, 3_000_000, 
 SERVING_SIZES = [(640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
 
 
-def serving(dev, sa, rate: float, seconds: float, warmup_s: float = 1.5, budget_gib: int = 96,
-            in_flight: int = 3, batch: int = 8, seed: int = 7) -> dict:
-    """Poisson stream of render requests over 8 resident clustered scenes
-    (0.3M..6M splats): uniform scene, random pose on the radius-6 sphere,
-    resolution uniform over 640x480..3840x2160.  Latency is the server's
-    submit -> seen-done time of every request that arrived in the measured
-    window; throughput is the frames finished in that window per second."""
-    from splatter_amd.synth import Workload, sphere_pose
+def serving_setup(dev, sa, budget_gib: int = 96, in_flight: int = 3, batch: int = 8):
+    """A server over 8 resident clustered scenes (0.3M..6M splats)."""
+    from splatter_amd.synth import Workload
     cfg = sa.Configuration(spherical_harmonics_order=3, max_splat_count=max(SERVING_SCENES), max_views=batch,
                            tile_pairs_per_splat=8)
     server = sa.Server(dev, cfg, max_in_flight=in_flight, max_width=3840, max_height=2160,
@@ -124,9 +119,19 @@ def serving(dev, sa, rate: float, seconds: float, warmup_s: float = 1.5, budget_
         scenes.append(sc)
     dev.synchronize()
     log(f"serving: {len(scenes)} scenes ({sum(SERVING_SCENES) / 1e6:.1f}M splats) resident in {time.perf_counter() - t0:.1f}s")
+    return server, scenes, {"memory_budget_gib": budget_gib, "max_in_flight": in_flight, "batch_width": batch}
+
+
+def serving(server, scenes, setup: dict, rate: float, seconds: float, warmup_s: float = 1.5, seed: int = 7) -> dict:
+    """Poisson stream of render requests over the resident scenes: uniform
+    scene, random pose on the radius-6 sphere, resolution uniform over
+    640x480..3840x2160.  Latency is the server's submit -> seen-done time of
+    every request that arrived in the measured window; throughput is the
+    frames finished in that window per second."""
+    from splatter_amd.synth import sphere_pose
     rng = np.random.default_rng(seed)
     poses = [sphere_pose(10_000 + i) for i in range(512)]
-    lat, sizes, batches = [], [], []
+    lat, batches = [], []
     done_in_window = 0
     failed = 0
     total = warmup_s + seconds
@@ -149,7 +154,6 @@ def serving(dev, sa, rate: float, seconds: float, warmup_s: float = 1.5, budget_
                 failed += 1
             elif a >= warmup_s:
                 lat.append(info["done_ms"] - info["submit_ms"])
-                sizes.append(info["width"] * info["height"])
                 batches.append(info["batch_views"])
             if warmup_s <= (time.perf_counter() - start) <= total:
                 done_in_window += 1
@@ -159,14 +163,12 @@ def serving(dev, sa, rate: float, seconds: float, warmup_s: float = 1.5, budget_
         if now > total + 30:
             raise RuntimeError("serving: requests still unfinished 30 s after the last arrival")
     st = server.stats()
-    server.close()
     lat = np.array(lat)
     return {"rate_per_s": rate, "seconds": seconds, "requests": int(len(lat)), "failed": failed,
             "frames_per_s": round(done_in_window / seconds, 2),
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_batch_views": round(float(np.mean(batches)), 2),
-            "memory_peak_gib": round(st["memory_peak"] / 2 ** 30, 2), "memory_budget_gib": budget_gib,
-            "slots": st["slots"], "max_in_flight": in_flight, "batch_width": batch,
+            "memory_peak_gib": round(st["memory_peak"] / 2 ** 30, 2), "slots": st["slots"], **setup,
             "scenes_m": [n / 1e6 for n in SERVING_SCENES], "sizes": [f"{w}x{h}" for w, h in SERVING_SIZES]}
 
 
@@ -204,6 +206,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="0 skips the serving measurement")
+    ap.add_argument("--serving-sweep", type=str, default="",
+                    help="comma-separated request rates: run only the serving measurement at each and print them")
     ap.add_argument("--profile", action="store_true",
                     help="batched steps and the stage-timing pass only (every launch has the batch's shape), "
                          "then three 256 MiB device-to-device copies whose counters scale FETCH_SIZE/WRITE_SIZE")
@@ -216,6 +220,16 @@ def main() -> int:
 
     import splatter_amd as sa
     from splatter_amd.synth import Workload, load_scene, orbit_poses
+
+    if args.serving_sweep:
+        dev = sa.Device(0)
+        server, scenes, setup = serving_setup(dev, sa)
+        for rate in [float(x) for x in args.serving_sweep.split(",")]:
+            res = serving(server, scenes, setup, rate, args.serving_seconds)
+            log(f"serving at {rate:g}/s: p50 {res['p50_ms']} ms, p99 {res['p99_ms']} ms, "
+                f"{res['frames_per_s']} frames/s, mean batch {res['mean_batch_views']}")
+            print(json.dumps(res), flush=True)
+        return 0
 
     W, H, V_PER = args.width, args.height, args.views
     L = 3
@@ -238,8 +252,7 @@ def main() -> int:
         # each candidate timed on the same resident scene
         from splatter_amd import autotune as at
         r.close()
-        tuned = at.autotune(sa, dev, cfg, lambda _r: scene, poses, W, H,
-                            cands=at.candidates(batch_views=(V_PER,)), steps=3, log=log)
+        tuned = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=3, log=log)
         tuned.apply(cfg)
         log(f"autotune picked {tuned.best.key()}")
         r = sa.Renderer(dev, cfg)
@@ -336,7 +349,9 @@ def main() -> int:
     if args.serving_seconds > 0:
         r.close()
         scene.close()
-        serve = serving(dev, sa, args.serving_rate, args.serving_seconds)
+        server, scenes, setup = serving_setup(dev, sa)
+        serve = serving(server, scenes, setup, args.serving_rate, args.serving_seconds)
+        server.close()
         log(f"serving at {args.serving_rate}/s: p50 {serve['p50_ms']} ms, p99 {serve['p99_ms']} ms, "
             f"{serve['frames_per_s']} frames/s")
 
@@ -361,6 +376,8 @@ def main() -> int:
                                f"256-pose orbit, {V_PER} views per batch",
                    "splats": args.splats, "width": W, "height": H, "views_per_batch": V_PER,
                    "radix_bits_per_digit": cfg.radix_bits_per_digit, "sort_tile_keys": cfg.sort_tile_keys,
+                   "raster_batch": cfg.raster_batch, "raster_longest_first": cfg.raster_longest_first,
+                   "raster_walk": cfg.raster_walk,
                    "autotune": tuned.to_json() if tuned else None},
         "plain_frames_per_s": round(plain_fps, 2),
         "batched_over_plain": round(batched_fps / plain_fps, 3),

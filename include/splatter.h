@@ -94,6 +94,8 @@ typedef struct spl_configuration {
     uint32_t sort_tile_keys;              /* keys per radix-sort tile: 4096 (256 lanes) or 8192 (512 lanes) */
     uint32_t raster_walk;                 /* 0: every lane tests each listed entry; 1: each lane walks the
                                              entries covering its pixel (same image) */
+    uint32_t raster_batch;                /* entries a raster block stages per batch: 128 or 256 */
+    uint32_t raster_longest_first;        /* 1: tiles with the longest lists start first (same image) */
 } spl_configuration;
 
 const char* spl_last_error(void);

@@ -72,6 +72,8 @@ class Configuration:
     use_hip_graphs: bool = True
     sort_tile_keys: int = 4096
     raster_walk: int = 0
+    raster_batch: int = 128
+    raster_longest_first: int = 0
 
     def to_c(self) -> SplConfiguration:
         c = SplConfiguration()
@@ -92,6 +94,8 @@ class Configuration:
         c.use_hip_graphs = int(bool(self.use_hip_graphs))
         c.sort_tile_keys = int(self.sort_tile_keys)
         c.raster_walk = int(self.raster_walk)
+        c.raster_batch = int(self.raster_batch)
+        c.raster_longest_first = int(self.raster_longest_first)
         return c
 
 

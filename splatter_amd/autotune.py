@@ -1,7 +1,8 @@
 """Autotuner for the batched form: picks the radix digit width, the radix-sort
-tile (keys per tile, i.e. the sort block's lane count), the rasteriser's walk
-(entry-major or per-pixel) and the batch width (views per launch sequence)
-for one scene, resolution and pose set by timing every candidate on the
+tile (keys per tile, i.e. the sort block's lane count), the rasteriser's
+batch (entries a tile block stages per round), walk (entry-major or
+per-pixel) and tile order, and the batch width (views per launch sequence)
+for one scene, resolution and pose set by timing the candidates on the
 device, and can keep the answer in a JSON cache.
 
 The candidates all render identical images and sort orders (the tests check
@@ -20,6 +21,8 @@ import numpy as np
 RADIX_BITS = (8, 10)
 SORT_TILE_KEYS = (4096, 8192)
 RASTER_WALK = (0, 1)
+RASTER_BATCH = (128, 256)
+RASTER_LONGEST_FIRST = (0, 1)
 BATCH_VIEWS = (8, 16, 32)
 
 
@@ -29,9 +32,17 @@ class Candidate:
     sort_tile_keys: int
     max_views: int
     raster_walk: int = 0
+    raster_batch: int = 128
+    raster_longest_first: int = 0
 
     def key(self) -> str:
-        return f"r{self.radix_bits_per_digit}-t{self.sort_tile_keys}-w{self.raster_walk}-v{self.max_views}"
+        return (f"r{self.radix_bits_per_digit}-t{self.sort_tile_keys}-b{self.raster_batch}-w{self.raster_walk}"
+                f"-o{self.raster_longest_first}-v{self.max_views}")
+
+    def replace(self, **kw) -> "Candidate":
+        d = dict(self.__dict__)
+        d.update(kw)
+        return Candidate(**d)
 
 
 @dataclass
@@ -45,6 +56,8 @@ class TuneResult:
         config.sort_tile_keys = self.best.sort_tile_keys
         config.max_views = self.best.max_views
         config.raster_walk = self.best.raster_walk
+        config.raster_batch = self.best.raster_batch
+        config.raster_longest_first = self.best.raster_longest_first
 
     def to_json(self) -> dict:
         return {"best": self.best.__dict__, "frames_per_s": self.frames_per_s, "table": self.table}
@@ -68,6 +81,8 @@ def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand
     cfg.sort_tile_keys = cand.sort_tile_keys
     cfg.max_views = cand.max_views
     cfg.raster_walk = cand.raster_walk
+    cfg.raster_batch = cand.raster_batch
+    cfg.raster_longest_first = cand.raster_longest_first
     r = sa.Renderer(device, cfg)
     try:
         scene = scene_loader(r)
@@ -126,3 +141,28 @@ def store_cached(path: str, key: str, result: TuneResult) -> None:
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as f:
         json.dump(d, f, indent=1, sort_keys=True)
+
+
+def autotune_staged(sa, device, base, scene_loader, poses, width: int, height: int, views: int, steps: int = 3,
+                    log=None, walk=(0,)) -> TuneResult:
+    """Coordinate search: the sort's digit width x tile with the raster at
+    its defaults, then the raster's batch x tile order (x walk) with the best
+    sort, at a fixed batch width."""
+    table = []
+
+    def run(cands):
+        best = None
+        for c in cands:
+            fps = measure(sa, device, base, scene_loader, poses, width, height, c, steps)
+            table.append((c.key(), round(fps, 2)))
+            if log:
+                log(f"autotune {c.key()}: {fps:.1f} frames/s")
+            if best is None or fps > best[1]:
+                best = (c, fps)
+        return best
+
+    start = Candidate(8, 4096, views, 0, base.raster_batch, 0)
+    c, _ = run([start.replace(radix_bits_per_digit=r, sort_tile_keys=t) for r in RADIX_BITS for t in SORT_TILE_KEYS])
+    c, fps = run([c.replace(raster_batch=b, raster_longest_first=o, raster_walk=w)
+                  for b in RASTER_BATCH for o in RASTER_LONGEST_FIRST for w in walk])
+    return TuneResult(c, round(fps, 2), table)

@@ -186,6 +186,72 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
   }
 }
 
+// Raster order of a view's tiles: the tiles with the longest pair lists
+// ("heavy": at least 8x the mean and 2048 pairs, at most heavy_cap of them,
+// whole log2 classes only) first, longest class first, then every other
+// tile in row-major order.  A heavy tile blends thousands of entries per
+// pixel, one after another; started last, it would outlast the rest of the
+// frame.
+__global__ __launch_bounds__(1024) void k_tile_order(OrderLaunch a) {
+  constexpr int THREADS = 1024;
+  __shared__ uint32_t s_cnt[32];
+  __shared__ uint32_t s_cur[32];
+  __shared__ uint32_t s_tmp[THREADS / kWave];
+  __shared__ uint32_t s_cut;
+  const uint32_t v = blockIdx.y, tid = threadIdx.x;
+  const uint32_t T = a.uniforms[v].tiles_x * a.uniforms[v].tiles_y;
+  const uint2* ranges = a.ranges + v * a.ranges_stride;
+  uint32_t* order = a.order + v * a.order_stride;
+  uint32_t* counters = a.counters + (size_t)v * kCounterWords;
+  uint32_t k = counters[kPairs];
+  if (k > a.pair_cap) k = a.pair_cap;
+  const uint32_t mean = T ? k / T : 0u;
+  const uint32_t thr = max(8u * mean, 2048u);
+  if (tid < 32) s_cnt[tid] = 0u;
+  __syncthreads();
+  const uint32_t chunk = (T + THREADS - 1) / THREADS;
+  const uint32_t t0 = min(tid * chunk, T), t1 = min(t0 + chunk, T);
+  auto cls = [&](uint32_t t) -> int {  // log2 class of a heavy candidate, -1 otherwise
+    const uint2 r = ranges[t];
+    const uint32_t len = r.y - r.x;
+    return len >= thr ? 31 - __clz(len) : -1;
+  };
+  for (uint32_t t = t0; t < t1; ++t) {
+    const int c = cls(t);
+    if (c >= 0) atomicAdd(&s_cnt[c], 1u);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // whole classes, longest first, while they fit under the cap
+    uint32_t h = 0, cut = 32;
+    for (int c = 31; c >= 0; --c) {
+      if (h + s_cnt[c] > a.heavy_cap) break;
+      s_cur[c] = h;
+      h += s_cnt[c];
+      cut = (uint32_t)c;
+    }
+    s_cut = cut;
+    counters[kHeavyTiles] = h;
+  }
+  __syncthreads();
+  const uint32_t cut = s_cut;
+  uint32_t light = 0;
+  for (uint32_t t = t0; t < t1; ++t) {
+    const int c = cls(t);
+    if (c >= (int)cut) order[atomicAdd(&s_cur[c], 1u)] = t;
+    else ++light;
+  }
+  uint32_t total;
+  uint32_t pos = block_exclusive_scan<THREADS>(light, s_tmp, total);
+  const uint32_t heavy = T - total;
+  for (uint32_t t = t0; t < t1; ++t)
+    if (cls(t) < (int)cut) order[heavy + pos++] = t;
+}
+
+void launch_tile_order(const OrderLaunch& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_order, dim3(1, a.views), dim3(1024), 0, s, a);
+}
+
 // One 256-lane block per 16x16 tile; wave w owns the 8x8 quadrant
 // (w & 1, w >> 1), one lane per pixel.  Entries arrive 256 at a time in draw
 // order: every lane gathers one record and stores the prepared entry (inverse
@@ -194,26 +260,54 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
 // then keeps, in order, only the entries whose pixel box meets its quadrant
 // (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
 // stops blending, and the block stops when all four have.
-template <bool UNALIGNED, bool F32, bool WALK>
+template <bool UNALIGNED, bool F32, bool WALK, int B>
 __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
-  constexpr int B = 256;
+  static_assert(B == 128 || B == 256, "entries staged per batch");
+  constexpr int U = B / 64;  // list slots per step of the entry-major walk
   __shared__ float4 s_m[2][B];     // i00 i01 i10 i11
   __shared__ float4 s_t[2][B];     // tx ty half alpha
   __shared__ float4 s_c[2][B];     // r g b -
   __shared__ uint2 s_box[2][B];    // pixel box lo / hi (x | y << 16)
   __shared__ __align__(16) uint8_t s_list[4][B];  // per-wave compacted entry slots, read 4 at a time
 
-  const uint32_t v = blockIdx.y;
+  // Block -> (view, tile).  First the heavy tiles of every view (their
+  // count rounded up to 8 blocks), then each view's other tiles in a slot
+  // range of slots_per_view blocks, banded per XCD: blocks are dealt
+  // round-robin to the 8 XCDs, and each XCD takes a contiguous run of rows
+  // (neighbouring tiles share records in its L2).
+  const uint32_t nv = (uint32_t)a.views;
+  uint32_t v = 0, pos = 0;
+  {
+    uint32_t heavy_total = 0;
+    if (a.longest_first)
+      for (uint32_t i = 0; i < nv; ++i) heavy_total += a.counters[(size_t)i * kCounterWords + kHeavyTiles];
+    const uint32_t heavy_blocks = (heavy_total + 7u) & ~7u;
+    uint32_t b = blockIdx.x;
+    if (b < heavy_blocks) {
+      if (b >= heavy_total) return;
+      for (;; ++v) {
+        const uint32_t h = a.counters[(size_t)v * kCounterWords + kHeavyTiles];
+        if (b < h) break;
+        b -= h;
+      }
+      pos = b;
+    } else {
+      b -= heavy_blocks;
+      v = b / a.slots_per_view;
+      if (v >= nv) return;
+      const uint32_t k = b % a.slots_per_view, per = a.slots_per_view / 8;
+      const uint32_t p = (k & 7u) * per + (k >> 3);
+      const uint32_t h = a.longest_first ? a.counters[(size_t)v * kCounterWords + kHeavyTiles] : 0u;
+      const uint32_t tv = a.uniforms[v].tiles_x * a.uniforms[v].tiles_y;
+      if (p >= tv - h) return;
+      pos = h + p;
+    }
+  }
   const Uniforms& u = a.uniforms[v];
-  const uint32_t T = u.tiles_x * u.tiles_y;
   uint2* ranges = a.ranges + v * a.ranges_stride;
   const uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
   const float4* records = a.records + v * a.rec_stride;
-  // XCD-aware order: blocks are dealt round-robin to the 8 XCDs, so give
-  // each XCD a contiguous band of tiles (neighbouring tiles share records).
-  const uint32_t per_xcd = gridDim.x / 8;
-  const uint32_t tile = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
-  if (tile >= T) return;
+  const uint32_t tile = a.longest_first ? a.order[v * a.order_stride + pos] : pos;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t qx0 = (tile % u.tiles_x) * kTileSize + (w & 1u) * 8u;
   const uint32_t qy0 = (tile / u.tiles_x) * kTileSize + (w >> 1) * 8u;
@@ -242,7 +336,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   // register stage of one entry per lane
   float4 g0 = make_float4(0, 0, 0, 0), g1 = g0, g2 = g0;
   auto fetch = [&](uint32_t base) {
-    if (base + tid < range.y) {
+    if (tid < (uint32_t)B && base + tid < range.y) {
       uint32_t c = pair_vals[base + tid];
       if (c >= visible) c = 0;  // only after an upstream error; never read past the records
       g0 = records[3 * (size_t)c + 0];
@@ -251,6 +345,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     }
   };
   auto stage = [&](int buf) {
+    if (tid >= (uint32_t)B) return;
     const float A00 = g1.x, A10 = g1.y, A01 = g1.z, A11 = g1.w;
     const float det = A00 * A11 - A10 * A01;
     const float rs = (1.0f / det) * kGaussScale;  // inverse map scaled by S
@@ -350,18 +445,20 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
       if constexpr (!WALK) {
         // entry-major: the wave steps through its list, every lane testing
         // its pixel; four list slots per step so their LDS reads go together
-        for (uint32_t i = 0; i < n; i += 4) {
-          const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][i]);
-          float4 m[4], t[4], col[4];
+        for (uint32_t i = 0; i < n; i += U) {
+          uint32_t js;
+          if constexpr (U == 4) js = *reinterpret_cast<const uint32_t*>(&s_list[w][i]);
+          else js = *reinterpret_cast<const uint16_t*>(&s_list[w][i]);
+          float4 m[U], t[U], col[U];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
+          for (int k = 0; k < U; ++k) {
             const uint32_t j = (js >> (8 * k)) & 255u;  // slots past n hold stale, in-range indices
             m[k] = s_m[buf][j];
             t[k] = s_t[buf][j];
             col[k] = s_c[buf][j];
           }
 #pragma unroll
-          for (int k = 0; k < 4; ++k) {
+          for (int k = 0; k < U; ++k) {
             if (i + k >= n) break;
             float u, w2;
             if (covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k]);
@@ -426,19 +523,25 @@ void launch_ranges(const RangesLaunch& a, hipStream_t s) {
   hipLaunchKernelGGL(k_tile_ranges, dim3(a.grid_x, a.views), dim3(256), 0, s, a);
 }
 
+template <bool UNALIGNED, bool F32, bool WALK>
+void launch_raster_as(bool small, dim3 grid, dim3 block, hipStream_t s, const RasterLaunch& a) {
+  if (small) hipLaunchKernelGGL((k_raster<UNALIGNED, F32, WALK, 128>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_raster<UNALIGNED, F32, WALK, 256>), grid, block, 0, s, a);
+}
+
 void launch_raster(const RasterLaunch& a, hipStream_t s) {
-  const dim3 grid(a.grid_x, a.views);
-  const dim3 block(256);
+  const dim3 grid(a.grid_x), block(256);
+  const bool small = a.batch_entries == 128;
   const int mode = (a.use_unaligned_rectangles ? 4 : 0) | (a.blend_f32 ? 2 : 0) | (a.walk ? 1 : 0);
   switch (mode) {
-    case 0: hipLaunchKernelGGL((k_raster<false, false, false>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((k_raster<false, false, true>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_raster<false, true, false>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_raster<false, true, true>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_raster<true, false, false>), grid, block, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((k_raster<true, false, true>), grid, block, 0, s, a); break;
-    case 6: hipLaunchKernelGGL((k_raster<true, true, false>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((k_raster<true, true, true>), grid, block, 0, s, a); break;
+    case 0: launch_raster_as<false, false, false>(small, grid, block, s, a); break;
+    case 1: launch_raster_as<false, false, true>(small, grid, block, s, a); break;
+    case 2: launch_raster_as<false, true, false>(small, grid, block, s, a); break;
+    case 3: launch_raster_as<false, true, true>(small, grid, block, s, a); break;
+    case 4: launch_raster_as<true, false, false>(small, grid, block, s, a); break;
+    case 5: launch_raster_as<true, false, true>(small, grid, block, s, a); break;
+    case 6: launch_raster_as<true, true, false>(small, grid, block, s, a); break;
+    default: launch_raster_as<true, true, true>(small, grid, block, s, a); break;
   }
 }
 

@@ -88,6 +88,7 @@ struct spl_renderer {
   uint32_t* depth_hist = nullptr;
   uint32_t* pair_hist = nullptr;
   uint2* ranges = nullptr;
+  uint32_t* tile_order = nullptr;  // per view, ranges_cap tiles
   uint32_t ranges_cap = 0;
   // shared
   uint32_t* vis = nullptr;
@@ -156,6 +157,7 @@ void renderer_free(spl_renderer* r) {
   dev_free(r->depth_hist);
   dev_free(r->pair_hist);
   dev_free(r->ranges);
+  dev_free(r->tile_order);
   dev_free(r->vis);
   dev_free(r->d_uniforms);
   dev_free(r->d_pixels);
@@ -223,14 +225,16 @@ int ensure_ranges(spl_renderer* r, uint32_t tiles) {
   HIP_TRY(hipStreamSynchronize(r->stream));
   drop_graph(r);
   dev_free(r->ranges);
-  r->bytes -= (uint64_t)r->ranges_cap * r->max_views * sizeof(uint2);
+  dev_free(r->tile_order);
+  r->bytes -= (uint64_t)r->ranges_cap * r->max_views * (sizeof(uint2) + sizeof(uint32_t));
   r->ranges_cap = 0;
   const size_t words = (size_t)tiles * r->max_views;
   HIP_TRY(dev_alloc(&r->ranges, words));
+  HIP_TRY(dev_alloc(&r->tile_order, words));
   HIP_TRY(hipMemsetAsync(r->ranges, 0, words * sizeof(uint2), r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
   r->ranges_cap = tiles;
-  r->bytes += (uint64_t)words * sizeof(uint2);
+  r->bytes += (uint64_t)words * (sizeof(uint2) + sizeof(uint32_t));
   return SPL_OK;
 }
 
@@ -423,6 +427,19 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.views = (int)nv;
     launch_ranges(a, q);
   }
+  const uint32_t slots = ((b.max_tiles + 7) / 8) * 8;
+  const uint32_t heavy_cap = std::max<uint32_t>(8, slots / 8);
+  if (cfg.raster_longest_first) {
+    OrderLaunch a{};
+    a.ranges = r->ranges; a.ranges_stride = r->ranges_cap;
+    a.uniforms = r->d_uniforms;
+    a.counters = r->counters;
+    a.pair_cap = (uint32_t)r->pair_cap;
+    a.order = r->tile_order; a.order_stride = r->ranges_cap;
+    a.heavy_cap = heavy_cap;
+    a.views = (int)nv;
+    launch_tile_order(a, q);
+  }
   mark(7);
   {
     RasterLaunch a{};
@@ -435,9 +452,14 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
     a.blend_f32 = cfg.blend_mode == SPL_BLEND_F32 ? 1 : 0;
     a.walk = cfg.raster_walk ? 1 : 0;
+    a.batch_entries = (int)cfg.raster_batch;
+    a.longest_first = cfg.raster_longest_first ? 1 : 0;
     a.bgra = cfg.frame_format == SPL_FRAME_FORMAT_BGRA8_UNORM ? 1 : 0;
     a.tile_work = b.tile_stats ? r->tile_work : nullptr;
-    a.grid_x = (int)(((b.max_tiles + 7) / 8) * 8);
+    a.order = r->tile_order; a.order_stride = r->ranges_cap;
+    a.slots_per_view = slots;
+    a.heavy_cap = heavy_cap;
+    a.grid_x = (int)(nv * slots + (cfg.raster_longest_first ? ((nv * heavy_cap + 7) / 8) * 8 : 0));
     a.views = (int)nv;
     launch_raster(a, q);
   }
@@ -582,6 +604,9 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   if (cfg->sort_tile_keys != 0 && cfg->sort_tile_keys != 4096 && cfg->sort_tile_keys != 8192)
     return fail(SPL_ERROR_UNSUPPORTED, "sort_tile_keys must be 4096 or 8192");
   if (cfg->raster_walk > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_walk must be 0 or 1");
+  if (cfg->raster_batch != 0 && cfg->raster_batch != 128 && cfg->raster_batch != 256)
+    return fail(SPL_ERROR_UNSUPPORTED, "raster_batch must be 128 or 256");
+  if (cfg->raster_longest_first > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_longest_first must be 0 or 1");
   HIP_TRY(hipSetDevice(dev->ordinal));
   auto* r = new (std::nothrow) spl_renderer();
   if (!r) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: host allocation");
@@ -603,6 +628,7 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   r->depth_bits = (int)cfg->radix_bits_per_digit;
   r->depth_passes = (30 + r->depth_bits - 1) / r->depth_bits;
   if (r->cfg.sort_tile_keys == 0) r->cfg.sort_tile_keys = 4096;
+  if (r->cfg.raster_batch == 0) r->cfg.raster_batch = 128;
   r->sort_threads = r->cfg.sort_tile_keys == 8192 ? kSortThreadsWide : kSortThreads;
   const int radix = 1 << r->depth_bits;
   // a multiple of 4: the tile-range pass reads the pair keys as uint4
@@ -640,7 +666,7 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   chk(alloc(&r->d_uniforms, V));
   chk(alloc(&r->d_pixels, V));
   if (dry) {
-    *dry_bytes = r->bytes + (uint64_t)presize_tiles * V * sizeof(uint2);
+    *dry_bytes = r->bytes + (uint64_t)presize_tiles * V * (sizeof(uint2) + sizeof(uint32_t));
     delete r;
     return SPL_OK;
   }

@@ -44,6 +44,7 @@ enum CounterSlot : int {
   kPairs = 33,
   kPairOverflow = 34,
   kScatterError = 35,
+  kHeavyTiles = 36, // tiles the raster takes first (longest lists)
   kCounterWords = 64
 };
 
@@ -148,7 +149,20 @@ struct RangesLaunch {
   int grid_x, views;
 };
 
+struct OrderLaunch {
+  const uint2* ranges; size_t ranges_stride;
+  const Uniforms* uniforms;
+  uint32_t* counters;
+  uint32_t pair_cap;
+  uint32_t* order; size_t order_stride;      // per view: heavy tiles, longest first, then the rest in order
+  uint32_t heavy_cap;                        // at most this many heavy tiles per view
+  int views;
+};
+
 struct RasterLaunch {
+  const uint32_t* order; size_t order_stride;
+  uint32_t slots_per_view;   // multiple of 8, >= the largest view's tile count
+  uint32_t heavy_cap;
   const uint32_t* pair_vals; size_t pair_stride;
   const float4* records; size_t rec_stride;
   uint2* ranges; size_t ranges_stride;
@@ -158,6 +172,8 @@ struct RasterLaunch {
   int use_unaligned_rectangles;
   int blend_f32;
   int walk;                  // 1: per-pixel walk of covering entries, 0: entry-major
+  int batch_entries;         // entries staged in LDS per batch: 128 or 256
+  int longest_first;         // take the tiles in the order k_tile_order wrote
   int bgra;
   uint2* tile_work;          // optional, view 0: per tile (list length, entries consumed)
   int grid_x, views;
@@ -169,6 +185,7 @@ void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s);
 void launch_project(const ProjectLaunch& a, hipStream_t s);
 void launch_bin(const BinLaunch& a, hipStream_t s);
 void launch_ranges(const RangesLaunch& a, hipStream_t s);
+void launch_tile_order(const OrderLaunch& a, hipStream_t s);
 void launch_raster(const RasterLaunch& a, hipStream_t s);
 
 }  // namespace spl

@@ -111,12 +111,14 @@ def test_depth_sorting_none_draws_in_index_order():
     check_case(case, conics=False)
 
 
-@pytest.mark.parametrize("bits,tile,walk", [(10, 4096, 0), (8, 8192, 0), (10, 8192, 1), (8, 4096, 1)])
-def test_autotuner_options(bits, tile, walk):
+@pytest.mark.parametrize("bits,tile,walk,batch,longest", [
+    (10, 4096, 0, 256, 0), (8, 8192, 0, 128, 1), (10, 8192, 1, 128, 0), (8, 4096, 1, 256, 1)])
+def test_autotuner_options(bits, tile, walk, batch, longest):
     # every option the autotuner may pick gives the same bit-exact order and image
-    say(f"radix bits {bits}, sort tile {tile}, raster walk {walk}")
-    case = Case(Workload("gaussian", 200_000, seed=5),
-                config(radix_bits_per_digit=bits, sort_tile_keys=tile, raster_walk=walk), 800, 600, sphere_pose(3))
+    say(f"radix bits {bits}, sort tile {tile}, raster walk {walk}, batch {batch}, longest first {longest}")
+    case = Case(Workload("clustered", 300_000, seed=5),
+                config(radix_bits_per_digit=bits, sort_tile_keys=tile, raster_walk=walk, raster_batch=batch,
+                       raster_longest_first=longest), 800, 600, sphere_pose(3))
     check_case(case, conics=False)
 
 
@@ -158,13 +160,14 @@ def test_deterministic_and_frame_size_changes():
     assert np.array_equal(a, b) and np.array_equal(ka, kb) and np.array_equal(ia, ib)
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_render_views_equals_single_frames(graphs):
+@pytest.mark.parametrize("graphs,longest", [(True, 0), (False, 0), (True, 1)])
+def test_render_views_equals_single_frames(graphs, longest):
     """Batched form: uneven frame sizes in one launch sequence, more views
     than max_views (two batches), each view equal to its single render and
     to the oracle (entries bit-exact, image within tolerance)."""
     w = Workload("clustered", 300_000, seed=8)
-    case = Case(w, config(max_views=4, use_hip_graphs=graphs), 640, 360, orbit_poses(6)[0])
+    case = Case(w, config(max_views=4, use_hip_graphs=graphs, raster_longest_first=longest), 640, 360,
+                orbit_poses(6)[0])
     poses = orbit_poses(6)
     sizes = [(640, 360), (320, 240), (1280, 720), (97, 61), (640, 360), (500, 500)]
     frames = [sa.Frame(device(), wd, ht) for wd, ht in sizes]
