@@ -91,7 +91,8 @@ typedef struct spl_configuration {
     uint32_t tile_pairs_per_splat;        /* capacity of (splat, screen tile) pairs, per max_splat_count */
     uint32_t max_views;                   /* views per batched launch (1..=32); workspace scales with it */
     int32_t use_hip_graphs;               /* replay each batch shape's launch sequence from a hipGraph */
-    uint32_t sort_tile_keys;              /* keys per radix-sort tile: 4096 (256 lanes) or 8192 (512 lanes) */
+    uint32_t sort_tile_keys;              /* keys per radix-sort tile: 4096, 8192 or 16384 (256, 512 or
+                                             1024 lanes) */
     uint32_t raster_walk;                 /* 0: every lane tests each listed entry; 1: each lane walks the
                                              entries covering its pixel (same image) */
     uint32_t raster_batch;                /* entries a raster block stages per batch: 128 or 256 */

@@ -19,7 +19,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 RADIX_BITS = (8, 10)
-SORT_TILE_KEYS = (4096, 8192)
+SORT_TILE_KEYS = (4096, 8192, 16384)
 RASTER_WALK = (0, 1)
 RASTER_BATCH = (128, 256)
 RASTER_LONGEST_FIRST = (0, 1)

@@ -264,7 +264,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   const int Q = b.pair_passes;
   const uint32_t cus = (uint32_t)r->dev->cu_count;
   const uint32_t per_view = std::max<uint32_t>(1, cus * 4 / nv);
-  // radix-sort tiles: two 512-lane or four 256-lane blocks per CU, shared
+  // radix-sort tiles: one 1024-lane, two 512-lane or four 256-lane blocks per CU, shared
   // by the views (which sort side by side, so their look-back chains overlap)
   const uint32_t sort_tile = (uint32_t)r->sort_threads * kSortItems;
   const uint32_t sort_per_view = std::max<uint32_t>(1, cus * 4 * kSortThreads / r->sort_threads / nv);
@@ -601,8 +601,9 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
     return fail(SPL_ERROR_INVALID_ARGUMENT, "field_of_view_y must be in (0, pi)");
   if (cfg->max_views > (uint32_t)kMaxViews)
     return fail(SPL_ERROR_INVALID_ARGUMENT, "max_views must be in 1..=32");
-  if (cfg->sort_tile_keys != 0 && cfg->sort_tile_keys != 4096 && cfg->sort_tile_keys != 8192)
-    return fail(SPL_ERROR_UNSUPPORTED, "sort_tile_keys must be 4096 or 8192");
+  if (cfg->sort_tile_keys != 0 && cfg->sort_tile_keys != 4096 && cfg->sort_tile_keys != 8192 &&
+      cfg->sort_tile_keys != 16384)
+    return fail(SPL_ERROR_UNSUPPORTED, "sort_tile_keys must be 4096, 8192 or 16384");
   if (cfg->raster_walk > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_walk must be 0 or 1");
   if (cfg->raster_batch != 0 && cfg->raster_batch != 128 && cfg->raster_batch != 256)
     return fail(SPL_ERROR_UNSUPPORTED, "raster_batch must be 128 or 256");
@@ -629,7 +630,8 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   r->depth_passes = (30 + r->depth_bits - 1) / r->depth_bits;
   if (r->cfg.sort_tile_keys == 0) r->cfg.sort_tile_keys = 4096;
   if (r->cfg.raster_batch == 0) r->cfg.raster_batch = 128;
-  r->sort_threads = r->cfg.sort_tile_keys == 8192 ? kSortThreadsWide : kSortThreads;
+  r->sort_threads = r->cfg.sort_tile_keys == 16384 ? kSortThreadsXL
+                    : r->cfg.sort_tile_keys == 8192 ? kSortThreadsWide : kSortThreads;
   const int radix = 1 << r->depth_bits;
   // a multiple of 4: the tile-range pass reads the pair keys as uint4
   r->pair_cap = std::min<uint64_t>(r->cap * (uint64_t)r->cfg.tile_pairs_per_splat, (1ull << 30) - (1ull << 20)) & ~3ull;

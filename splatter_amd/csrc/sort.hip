@@ -8,7 +8,7 @@
 //    bumped by the lowest peer) carries the rank across the wave's items.
 //    Items are visited in input order, so the pass is stable -- the
 //    reference ranks with LDS atomicAdd (shaders.wgsl:428-429), which is not.
-//  - tiles are TILE = THREADS*ITEMS keys (4096 or 8192) taken in order from an
+//  - tiles are TILE = THREADS*ITEMS keys (4096, 8192 or 16384) taken in order from an
 //    atomic counter by a persistent grid (reference: 1024-key tiles, one
 //    workgroup each, shaders.wgsl:406-420).
 //  - the global digit offsets come from the upsweep histogram and are
@@ -59,12 +59,12 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
   const int shift = a.shift;
 
   {  // global exclusive digit starts
-    uint32_t v[DPT], sum = 0;
+    uint32_t hv[DPT], sum = 0;
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
       const uint32_t d = tid * DPT + k;
-      v[k] = d < (uint32_t)RADIX ? hist[d] : 0u;
-      sum += v[k];
+      hv[k] = d < (uint32_t)RADIX ? hist[d] : 0u;
+      sum += hv[k];
     }
     uint32_t total;
     uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     for (int k = 0; k < DPT; ++k) {
       const uint32_t d = tid * DPT + k;
       if (d < (uint32_t)RADIX) s_digit_start[d] = excl;
-      excl += v[k];
+      excl += hv[k];
     }
   }
 
@@ -237,7 +237,12 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
 
 void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s) {
   const dim3 grid(a.grid_x, a.views);
-  if (a.threads == kSortThreadsWide) {
+  if (a.threads == kSortThreadsXL) {
+    if (bits == 10)
+      hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreadsXL, kSortItems>), grid, dim3(kSortThreadsXL), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreadsXL, kSortItems>), grid, dim3(kSortThreadsXL), 0, s, a);
+  } else if (a.threads == kSortThreadsWide) {
     if (bits == 10)
       hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreadsWide, kSortItems>), grid, dim3(kSortThreadsWide), 0, s, a);
     else

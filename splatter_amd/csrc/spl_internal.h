@@ -60,6 +60,7 @@ struct SceneView {
 constexpr int kCullThreads = 256, kCullItems = 16;   // 4096 splats per cull / project tile
 constexpr int kSortThreads = 256, kSortItems = 16;   // 4096 keys per sort tile (the narrow form)
 constexpr int kSortThreadsWide = 512;                // 8192 keys per sort tile
+constexpr int kSortThreadsXL = 1024;                 // 16384 keys per sort tile
 constexpr int kBinThreads = 256, kBinItems = 4;      // 1024 draw ranks per bin tile
 constexpr int kCullTile = kCullThreads * kCullItems;
 
@@ -103,7 +104,7 @@ struct SortPassLaunch {
   int tile_slot;             // counter slot of this pass's tile counter
   uint32_t* clear_a; size_t clear_a_words; size_t clear_a_stride;
   uint32_t* clear_b; size_t clear_b_words; size_t clear_b_stride;
-  int threads;               // kSortThreads or kSortThreadsWide lanes per tile of kSortItems keys each
+  int threads;               // kSortThreads, kSortThreadsWide or kSortThreadsXL lanes, kSortItems keys each
   int grid_x, views;
 };
 

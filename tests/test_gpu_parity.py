@@ -112,7 +112,8 @@ def test_depth_sorting_none_draws_in_index_order():
 
 
 @pytest.mark.parametrize("bits,tile,walk,batch,longest", [
-    (10, 4096, 0, 256, 0), (8, 8192, 0, 128, 1), (10, 8192, 1, 128, 0), (8, 4096, 1, 256, 1)])
+    (10, 4096, 0, 256, 0), (8, 8192, 0, 128, 1), (10, 8192, 1, 128, 0), (8, 4096, 1, 256, 1),
+    (8, 16384, 0, 128, 0), (10, 16384, 0, 256, 1)])
 def test_autotuner_options(bits, tile, walk, batch, longest):
     # every option the autotuner may pick gives the same bit-exact order and image
     say(f"radix bits {bits}, sort tile {tile}, raster walk {walk}, batch {batch}, longest first {longest}")
