@@ -377,7 +377,7 @@ def main() -> int:
                    "splats": args.splats, "width": W, "height": H, "views_per_batch": V_PER,
                    "radix_bits_per_digit": cfg.radix_bits_per_digit, "sort_tile_keys": cfg.sort_tile_keys,
                    "raster_batch": cfg.raster_batch, "raster_longest_first": cfg.raster_longest_first,
-                   "raster_walk": cfg.raster_walk,
+                   "raster_mode": cfg.raster_mode,
                    "autotune": tuned.to_json() if tuned else None},
         "plain_frames_per_s": round(plain_fps, 2),
         "batched_over_plain": round(batched_fps / plain_fps, 3),

@@ -93,8 +93,10 @@ typedef struct spl_configuration {
     int32_t use_hip_graphs;               /* replay each batch shape's launch sequence from a hipGraph */
     uint32_t sort_tile_keys;              /* keys per radix-sort tile: 4096, 8192 or 16384 (256, 512 or
                                              1024 lanes) */
-    uint32_t raster_walk;                 /* 0: every lane tests each listed entry; 1: each lane walks the
-                                             entries covering its pixel (same image) */
+    uint32_t raster_mode;                 /* 0: each wave's lanes test every entry of its 8x8 quadrant;
+                                             1: each lane walks the entries covering its pixel;
+                                             2: four lane groups step through the lists of their 4x4
+                                             sub-blocks side by side (all the same image) */
     uint32_t raster_batch;                /* entries a raster block stages per batch: 128 or 256 */
     uint32_t raster_longest_first;        /* 1: tiles with the longest lists start first (same image) */
 } spl_configuration;

@@ -71,7 +71,7 @@ class Configuration:
     max_views: int = 1
     use_hip_graphs: bool = True
     sort_tile_keys: int = 4096
-    raster_walk: int = 0
+    raster_mode: int = 0
     raster_batch: int = 128
     raster_longest_first: int = 0
 
@@ -93,7 +93,7 @@ class Configuration:
         c.max_views = int(self.max_views)
         c.use_hip_graphs = int(bool(self.use_hip_graphs))
         c.sort_tile_keys = int(self.sort_tile_keys)
-        c.raster_walk = int(self.raster_walk)
+        c.raster_mode = int(self.raster_mode)
         c.raster_batch = int(self.raster_batch)
         c.raster_longest_first = int(self.raster_longest_first)
         return c

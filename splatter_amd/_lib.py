@@ -32,7 +32,7 @@ class SplConfiguration(C.Structure):
         ("max_views", C.c_uint32),
         ("use_hip_graphs", C.c_int32),
         ("sort_tile_keys", C.c_uint32),
-        ("raster_walk", C.c_uint32),
+        ("raster_mode", C.c_uint32),
         ("raster_batch", C.c_uint32),
         ("raster_longest_first", C.c_uint32),
     ]

@@ -20,7 +20,7 @@ import numpy as np
 
 RADIX_BITS = (8, 10)
 SORT_TILE_KEYS = (4096, 8192, 16384)
-RASTER_WALK = (0, 1)
+RASTER_MODE = (0, 1, 2)
 RASTER_BATCH = (128, 256)
 RASTER_LONGEST_FIRST = (0, 1)
 BATCH_VIEWS = (8, 16, 32)
@@ -31,12 +31,12 @@ class Candidate:
     radix_bits_per_digit: int
     sort_tile_keys: int
     max_views: int
-    raster_walk: int = 0
+    raster_mode: int = 0
     raster_batch: int = 128
     raster_longest_first: int = 0
 
     def key(self) -> str:
-        return (f"r{self.radix_bits_per_digit}-t{self.sort_tile_keys}-b{self.raster_batch}-w{self.raster_walk}"
+        return (f"r{self.radix_bits_per_digit}-t{self.sort_tile_keys}-b{self.raster_batch}-m{self.raster_mode}"
                 f"-o{self.raster_longest_first}-v{self.max_views}")
 
     def replace(self, **kw) -> "Candidate":
@@ -55,7 +55,7 @@ class TuneResult:
         config.radix_bits_per_digit = self.best.radix_bits_per_digit
         config.sort_tile_keys = self.best.sort_tile_keys
         config.max_views = self.best.max_views
-        config.raster_walk = self.best.raster_walk
+        config.raster_mode = self.best.raster_mode
         config.raster_batch = self.best.raster_batch
         config.raster_longest_first = self.best.raster_longest_first
 
@@ -68,9 +68,9 @@ class TuneResult:
 
 
 def candidates(radix_bits=RADIX_BITS, sort_tile_keys=SORT_TILE_KEYS, batch_views=BATCH_VIEWS,
-               raster_walk=RASTER_WALK):
+               raster_mode=RASTER_MODE):
     return [Candidate(r, t, v, w) for r, t, v, w in
-            itertools.product(radix_bits, sort_tile_keys, batch_views, raster_walk)]
+            itertools.product(radix_bits, sort_tile_keys, batch_views, raster_mode)]
 
 
 def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand: Candidate,
@@ -80,7 +80,7 @@ def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand
     cfg.radix_bits_per_digit = cand.radix_bits_per_digit
     cfg.sort_tile_keys = cand.sort_tile_keys
     cfg.max_views = cand.max_views
-    cfg.raster_walk = cand.raster_walk
+    cfg.raster_mode = cand.raster_mode
     cfg.raster_batch = cand.raster_batch
     cfg.raster_longest_first = cand.raster_longest_first
     r = sa.Renderer(device, cfg)
@@ -163,6 +163,6 @@ def autotune_staged(sa, device, base, scene_loader, poses, width: int, height: i
 
     start = Candidate(8, 4096, views, 0, base.raster_batch, 0)
     c, _ = run([start.replace(radix_bits_per_digit=r, sort_tile_keys=t) for r in RADIX_BITS for t in SORT_TILE_KEYS])
-    c, fps = run([c.replace(raster_batch=b, raster_longest_first=o, raster_walk=w)
+    c, fps = run([c.replace(raster_batch=b, raster_longest_first=o, raster_mode=w)
                   for b in RASTER_BATCH for o in RASTER_LONGEST_FIRST for w in walk])
     return TuneResult(c, round(fps, 2), table)

@@ -260,15 +260,17 @@ void launch_tile_order(const OrderLaunch& a, hipStream_t s) {
 // then keeps, in order, only the entries whose pixel box meets its quadrant
 // (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
 // stops blending, and the block stops when all four have.
-template <bool UNALIGNED, bool F32, bool WALK, int B>
+template <bool UNALIGNED, bool F32, int MODE, int B>
 __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   static_assert(B == 128 || B == 256, "entries staged per batch");
-  constexpr int U = B / 64;  // list slots per step of the entry-major walk
+  constexpr int U = B / 64;  // list slots per step of the entry-major walks
+  constexpr bool WALK = MODE == 1, SUBS = MODE == 2;
   __shared__ float4 s_m[2][B];     // i00 i01 i10 i11
   __shared__ float4 s_t[2][B];     // tx ty half alpha
   __shared__ float4 s_c[2][B];     // r g b -
   __shared__ uint2 s_box[2][B];    // pixel box lo / hi (x | y << 16)
   __shared__ __align__(16) uint8_t s_list[4][B];  // per-wave compacted entry slots, read 4 at a time
+  __shared__ __align__(16) uint8_t s_sub[SUBS ? 4 : 1][SUBS ? 4 : 1][SUBS ? B : 4];  // mode 2: per 4x4 sub-block
 
   // Block -> (view, tile).  First the heavy tiles of every view (their
   // count rounded up to 8 blocks), then each view's other tiles in a slot
@@ -311,7 +313,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const uint32_t qx0 = (tile % u.tiles_x) * kTileSize + (w & 1u) * 8u;
   const uint32_t qy0 = (tile / u.tiles_x) * kTileSize + (w >> 1) * 8u;
-  const uint32_t px = qx0 + (lane & 7u), py = qy0 + (lane >> 3);
+  // mode 2: lane group g = lane / 16 owns the 4x4 sub-block (g & 1, g >> 1)
+  const uint32_t g = lane >> 4;
+  const uint32_t px = SUBS ? qx0 + (g & 1u) * 4u + (lane & 3u) : qx0 + (lane & 7u);
+  const uint32_t py = SUBS ? qy0 + (g >> 1) * 4u + ((lane >> 2) & 3u) : qy0 + (lane >> 3);
   const uint32_t W = u.image_size[0], H = u.image_size[1];
   const bool inside = px < W && py < H;
   const uint2 range = ranges[tile];
@@ -329,6 +334,26 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
                                   (1.0f - (((float)qy0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
   constexpr bool f32 = F32;
   constexpr bool unaligned = UNALIGNED;
+  // Can entry (m, t) cover a pixel centre of the box [lox, hix] x [loy, hiy]
+  // (view plane)?  The tested coordinate pair is affine in the centre, so its
+  // exact range over the box sits at the corners; the slack is far above the
+  // rounding of the per-pixel evaluation.
+  auto range_meets = [&](const float4& m, const float4& t, float lox, float hix, float loy, float hiy) -> bool {
+    const float dxl = lox - t.x, dxh = hix - t.x, dyl = loy - t.y, dyh = hiy - t.y;
+    float ax0, ax1, ay0, ay1;  // range of the tested coordinate pair
+    if constexpr (unaligned) {
+      ax0 = fminf(m.x * dxl, m.x * dxh) + fminf(m.y * dyl, m.y * dyh);
+      ax1 = fmaxf(m.x * dxl, m.x * dxh) + fmaxf(m.y * dyl, m.y * dyh);
+      ay0 = fminf(m.z * dxl, m.z * dxh) + fminf(m.w * dyl, m.w * dyh);
+      ay1 = fmaxf(m.z * dxl, m.z * dxh) + fmaxf(m.w * dyl, m.w * dyh);
+    } else {
+      ax0 = dxl; ax1 = dxh; ay0 = dyl; ay1 = dyh;
+    }
+    const float h = t.z;
+    const float sx = 1e-4f * fmaxf(fmaxf(fabsf(ax0), fabsf(ax1)), h) + 1e-30f;
+    const float sy = 1e-4f * fmaxf(fmaxf(fabsf(ay0), fabsf(ay1)), h) + 1e-30f;
+    return ax1 + sx >= -h && ax0 - sx < h && ay1 + sy >= -h && ay0 - sy < h;
+  };
   const float margin = u.ellipse_margin;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = f32 ? 1.0f : 255.0f;
   const uint32_t visible = a.counters[(size_t)v * kCounterWords + kVisible];
@@ -378,6 +403,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
       // the coverage coordinate over the quadrant's pixel centres (it is
       // affine in the centre, so its extremes sit at the corner centres)
       uint32_t n = 0;
+      uint32_t ns[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
       for (int k = 0; k < B / 64; ++k) {
         const uint32_t j = k * 64 + lane;
@@ -386,29 +412,36 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           const uint2 bx = s_box[buf][j];
           const uint32_t x0 = bx.x & 0xFFFFu, y0 = bx.x >> 16, x1 = bx.y & 0xFFFFu, y1 = bx.y >> 16;
           ov = bx.x != 0xFFFFFFFFu && x0 <= qx0 + 7u && x1 >= qx0 && y0 <= qy0 + 7u && y1 >= qy0;
-          if (ov) {
-            const float4 m = s_m[buf][j];
-            const float4 t = s_t[buf][j];
-            const float dxl = q_lo.x - t.x, dxh = q_hi.x - t.x, dyl = q_lo.y - t.y, dyh = q_hi.y - t.y;
-            float ax0, ax1, ay0, ay1;  // range of the tested coordinate pair
-            if constexpr (unaligned) {
-              ax0 = fminf(m.x * dxl, m.x * dxh) + fminf(m.y * dyl, m.y * dyh);
-              ax1 = fmaxf(m.x * dxl, m.x * dxh) + fmaxf(m.y * dyl, m.y * dyh);
-              ay0 = fminf(m.z * dxl, m.z * dxh) + fminf(m.w * dyl, m.w * dyh);
-              ay1 = fmaxf(m.z * dxl, m.z * dxh) + fmaxf(m.w * dyl, m.w * dyh);
-            } else {
-              ax0 = dxl; ax1 = dxh; ay0 = dyl; ay1 = dyh;
+          if (ov && !SUBS) ov = range_meets(s_m[buf][j], s_t[buf][j], q_lo.x, q_hi.x, q_lo.y, q_hi.y);
+        }
+        if constexpr (!SUBS) {
+          const uint64_t bm = __ballot(ov);
+          if (ov) s_list[w][n + lanes_below(bm)] = (uint8_t)j;
+          n += (uint32_t)__popcll(bm);
+        } else {
+          // the same exact test per 4x4 sub-block, one list each
+#pragma unroll
+          for (int sb = 0; sb < 4; ++sb) {
+            bool o = ov;
+            if (o) {
+              const uint32_t bx0 = qx0 + (sb & 1) * 4u, by0 = qy0 + (sb >> 1) * 4u;
+              const uint2 bx = s_box[buf][j];
+              o = bx0 < W && by0 < H && (bx.x & 0xFFFFu) <= bx0 + 3u && (bx.y & 0xFFFFu) >= bx0 &&
+                  (bx.x >> 16) <= by0 + 3u && (bx.y >> 16) >= by0;
+              if (o) {
+                const uint32_t bx1 = min(bx0 + 3u, W - 1u), by1 = min(by0 + 3u, H - 1u);
+                const float lox = ((((float)bx0 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
+                const float hix = ((((float)bx1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
+                const float loy = (1.0f - (((float)by1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
+                const float hiy = (1.0f - (((float)by0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
+                o = range_meets(s_m[buf][j], s_t[buf][j], lox, hix, loy, hiy);
+              }
             }
-            // slack far above the rounding of the per-pixel evaluation
-            const float h = t.z;
-            const float sx = 1e-4f * fmaxf(fmaxf(fabsf(ax0), fabsf(ax1)), h) + 1e-30f;
-            const float sy = 1e-4f * fmaxf(fmaxf(fabsf(ay0), fabsf(ay1)), h) + 1e-30f;
-            ov = ax1 + sx >= -h && ax0 - sx < h && ay1 + sy >= -h && ay0 - sy < h;
+            const uint64_t bm = __ballot(o);
+            if (o) s_sub[w][sb][ns[sb] + lanes_below(bm)] = (uint8_t)j;
+            ns[sb] += (uint32_t)__popcll(bm);
           }
         }
-        const uint64_t bm = __ballot(ov);
-        if (ov) s_list[w][n + lanes_below(bm)] = (uint8_t)j;
-        n += (uint32_t)__popcll(bm);
       }
       __builtin_amdgcn_wave_barrier();
       // one fragment of entry slot j at this lane's pixel: coverage, alpha
@@ -442,7 +475,30 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           }
         }
       };
-      if constexpr (!WALK) {
+      if constexpr (SUBS) {
+        // entry-major per 4x4 sub-block: the four lane groups step through
+        // their own lists side by side, so one step blends up to four entries
+        const uint32_t mine = g == 0 ? ns[0] : g == 1 ? ns[1] : g == 2 ? ns[2] : ns[3];
+        const uint32_t most = max(max(ns[0], ns[1]), max(ns[2], ns[3]));
+        for (uint32_t i = 0; i < most; i += U) {
+          uint32_t js;
+          if constexpr (U == 4) js = *reinterpret_cast<const uint32_t*>(&s_sub[w][g][i]);
+          else js = *reinterpret_cast<const uint16_t*>(&s_sub[w][g][i]);
+          float4 m[U], t[U], col[U];
+#pragma unroll
+          for (int k = 0; k < U; ++k) {
+            const uint32_t j = (js >> (8 * k)) & 255u;  // slots past the list hold stale, in-range indices
+            m[k] = s_m[buf][j];
+            t[k] = s_t[buf][j];
+            col[k] = s_c[buf][j];
+          }
+#pragma unroll
+          for (int k = 0; k < U; ++k) {
+            float u, w2;
+            if (i + k < mine && covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k]);
+          }
+        }
+      } else if constexpr (!WALK) {
         // entry-major: the wave steps through its list, every lane testing
         // its pixel; four list slots per step so their LDS reads go together
         for (uint32_t i = 0; i < n; i += U) {
@@ -523,25 +579,28 @@ void launch_ranges(const RangesLaunch& a, hipStream_t s) {
   hipLaunchKernelGGL(k_tile_ranges, dim3(a.grid_x, a.views), dim3(256), 0, s, a);
 }
 
-template <bool UNALIGNED, bool F32, bool WALK>
+template <bool UNALIGNED, bool F32, int MODE>
 void launch_raster_as(bool small, dim3 grid, dim3 block, hipStream_t s, const RasterLaunch& a) {
-  if (small) hipLaunchKernelGGL((k_raster<UNALIGNED, F32, WALK, 128>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((k_raster<UNALIGNED, F32, WALK, 256>), grid, block, 0, s, a);
+  if (small) hipLaunchKernelGGL((k_raster<UNALIGNED, F32, MODE, 128>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_raster<UNALIGNED, F32, MODE, 256>), grid, block, 0, s, a);
+}
+
+template <bool UNALIGNED, bool F32>
+void launch_raster_mode(bool small, dim3 grid, dim3 block, hipStream_t s, const RasterLaunch& a) {
+  if (a.walk == 2) launch_raster_as<UNALIGNED, F32, 2>(small, grid, block, s, a);
+  else if (a.walk == 1) launch_raster_as<UNALIGNED, F32, 1>(small, grid, block, s, a);
+  else launch_raster_as<UNALIGNED, F32, 0>(small, grid, block, s, a);
 }
 
 void launch_raster(const RasterLaunch& a, hipStream_t s) {
   const dim3 grid(a.grid_x), block(256);
   const bool small = a.batch_entries == 128;
-  const int mode = (a.use_unaligned_rectangles ? 4 : 0) | (a.blend_f32 ? 2 : 0) | (a.walk ? 1 : 0);
-  switch (mode) {
-    case 0: launch_raster_as<false, false, false>(small, grid, block, s, a); break;
-    case 1: launch_raster_as<false, false, true>(small, grid, block, s, a); break;
-    case 2: launch_raster_as<false, true, false>(small, grid, block, s, a); break;
-    case 3: launch_raster_as<false, true, true>(small, grid, block, s, a); break;
-    case 4: launch_raster_as<true, false, false>(small, grid, block, s, a); break;
-    case 5: launch_raster_as<true, false, true>(small, grid, block, s, a); break;
-    case 6: launch_raster_as<true, true, false>(small, grid, block, s, a); break;
-    default: launch_raster_as<true, true, true>(small, grid, block, s, a); break;
+  if (a.use_unaligned_rectangles) {
+    if (a.blend_f32) launch_raster_mode<true, true>(small, grid, block, s, a);
+    else launch_raster_mode<true, false>(small, grid, block, s, a);
+  } else {
+    if (a.blend_f32) launch_raster_mode<false, true>(small, grid, block, s, a);
+    else launch_raster_mode<false, false>(small, grid, block, s, a);
   }
 }
 

@@ -451,7 +451,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.counters = r->counters;
     a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
     a.blend_f32 = cfg.blend_mode == SPL_BLEND_F32 ? 1 : 0;
-    a.walk = cfg.raster_walk ? 1 : 0;
+    a.walk = (int)cfg.raster_mode;
     a.batch_entries = (int)cfg.raster_batch;
     a.longest_first = cfg.raster_longest_first ? 1 : 0;
     a.bgra = cfg.frame_format == SPL_FRAME_FORMAT_BGRA8_UNORM ? 1 : 0;
@@ -604,7 +604,7 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   if (cfg->sort_tile_keys != 0 && cfg->sort_tile_keys != 4096 && cfg->sort_tile_keys != 8192 &&
       cfg->sort_tile_keys != 16384)
     return fail(SPL_ERROR_UNSUPPORTED, "sort_tile_keys must be 4096, 8192 or 16384");
-  if (cfg->raster_walk > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_walk must be 0 or 1");
+  if (cfg->raster_mode > 2) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_mode must be 0, 1 or 2");
   if (cfg->raster_batch != 0 && cfg->raster_batch != 128 && cfg->raster_batch != 256)
     return fail(SPL_ERROR_UNSUPPORTED, "raster_batch must be 128 or 256");
   if (cfg->raster_longest_first > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_longest_first must be 0 or 1");

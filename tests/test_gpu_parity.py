@@ -111,22 +111,30 @@ def test_depth_sorting_none_draws_in_index_order():
     check_case(case, conics=False)
 
 
-@pytest.mark.parametrize("bits,tile,walk,batch,longest", [
+@pytest.mark.parametrize("bits,tile,mode,batch,longest", [
     (10, 4096, 0, 256, 0), (8, 8192, 0, 128, 1), (10, 8192, 1, 128, 0), (8, 4096, 1, 256, 1),
-    (8, 16384, 0, 128, 0), (10, 16384, 0, 256, 1)])
-def test_autotuner_options(bits, tile, walk, batch, longest):
+    (8, 16384, 2, 128, 0), (10, 16384, 2, 256, 1)])
+def test_autotuner_options(bits, tile, mode, batch, longest):
     # every option the autotuner may pick gives the same bit-exact order and image
-    say(f"radix bits {bits}, sort tile {tile}, raster walk {walk}, batch {batch}, longest first {longest}")
+    say(f"radix bits {bits}, sort tile {tile}, raster mode {mode}, batch {batch}, longest first {longest}")
     case = Case(Workload("clustered", 300_000, seed=5),
-                config(radix_bits_per_digit=bits, sort_tile_keys=tile, raster_walk=walk, raster_batch=batch,
+                config(radix_bits_per_digit=bits, sort_tile_keys=tile, raster_mode=mode, raster_batch=batch,
                        raster_longest_first=longest), 800, 600, sphere_pose(3))
     check_case(case, conics=False)
 
 
-@pytest.mark.parametrize("unaligned,blend", [(True, sa.BlendMode.F32), (False, sa.BlendMode.UNORM8)])
-def test_raster_walk_other_modes(unaligned, blend):
+@pytest.mark.parametrize("mode,unaligned,blend", [(1, True, sa.BlendMode.F32), (1, False, sa.BlendMode.UNORM8),
+                                                  (2, True, sa.BlendMode.F32), (2, False, sa.BlendMode.UNORM8)])
+def test_raster_modes_other_rectangles_and_blends(mode, unaligned, blend):
     case = Case(Workload("clustered", 300_000, seed=8),
-                config(raster_walk=1, use_unaligned_rectangles=unaligned, blend_mode=blend), 640, 480, sphere_pose(8))
+                config(raster_mode=mode, use_unaligned_rectangles=unaligned, blend_mode=blend), 640, 480,
+                sphere_pose(8))
+    check_case(case, conics=False)
+
+
+def test_wide_frame_uses_pixel_boxes():
+    # more than 255 screen tiles across: binning reads 8-byte pixel boxes
+    case = Case(Workload("gaussian", 100_000, seed=9), config(), 4200, 96, sphere_pose(9, radius=3))
     check_case(case, conics=False)
 
 
