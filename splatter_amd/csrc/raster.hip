@@ -325,13 +325,6 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
 
   const float qx = ((((float)px + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
   const float qy = (1.0f - (((float)py + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
-  // view-plane extremes of the quadrant's pixel centres (x grows with px,
-  // y falls with py); only centres inside the image count
-  const uint32_t qx1 = min(qx0 + 7u, W - 1u), qy1 = min(qy0 + 7u, H - 1u);
-  const float2 q_lo = make_float2(((((float)qx0 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
-                                  (1.0f - (((float)qy1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
-  const float2 q_hi = make_float2(((((float)qx1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
-                                  (1.0f - (((float)qy0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
   constexpr bool f32 = F32;
   constexpr bool unaligned = UNALIGNED;
   // Can entry (m, t) cover a pixel centre of the box [lox, hix] x [loy, hiy]
@@ -384,7 +377,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     s_box[buf][tid] = make_uint2(__float_as_uint(g2.z), __float_as_uint(g2.w));
   };
 
-  bool wave_done = __ballot(inside) == 0ull;
+  // lanes whose pixel still takes fragments (inside the image, not yet
+  // opaque); in modes 0 and 1 lane y * 8 + x holds pixel (x, y) of the quadrant
+  uint64_t alive = __ballot(inside);
+  bool wave_done = alive == 0ull;
   uint32_t consumed = 0;
   int buf = 0;
   if (range.x < range.y) {
@@ -398,10 +394,30 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     if (more) fetch(base + B);
     consumed += cnt;
     if (!wave_done) {
-      // entries of this batch that can cover a pixel centre of the quadrant,
-      // in order: pixel box against the quadrant, then the exact range of
-      // the coverage coordinate over the quadrant's pixel centres (it is
-      // affine in the centre, so its extremes sit at the corner centres)
+      // Entries of this batch that can cover a pixel centre still taking
+      // fragments, in order: pixel box against the box of the live pixels,
+      // then the exact range of the coverage coordinate over that box's
+      // pixel centres (it is affine in the centre, so its extremes sit at the
+      // corner centres).  An entry that meets only opaque pixels would add
+      // s * 0 to them, so leaving it out changes nothing.  The live box's
+      // columns come from the OR of the mask's eight rows, its rows from the
+      // lowest and highest set bits.  Mode 2 lays its lanes out per sub-block
+      // (bits 16 g .. 16 g + 15 are sub-block g): it keeps the quadrant's box
+      // and skips opaque sub-blocks instead.
+      uint32_t ax0 = qx0, ax1 = min(qx0 + 7u, W - 1u), ay0 = qy0, ay1 = min(qy0 + 7u, H - 1u);
+      if constexpr (!SUBS) {
+        uint32_t cols = (uint32_t)alive | (uint32_t)(alive >> 32);
+        cols |= cols >> 16;
+        cols |= cols >> 8;
+        ax0 = qx0 + (uint32_t)__builtin_ctz(cols & 0xFFu);
+        ax1 = qx0 + 31u - (uint32_t)__builtin_clz(cols & 0xFFu);
+        ay0 = qy0 + ((uint32_t)__builtin_ctzll(alive) >> 3);
+        ay1 = qy0 + ((63u - (uint32_t)__builtin_clzll(alive)) >> 3);
+      }
+      const float2 q_lo = make_float2(((((float)ax0 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
+                                      (1.0f - (((float)ay1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
+      const float2 q_hi = make_float2(((((float)ax1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
+                                      (1.0f - (((float)ay0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
       uint32_t n = 0;
       uint32_t ns[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -411,7 +427,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
         if (j < cnt) {
           const uint2 bx = s_box[buf][j];
           const uint32_t x0 = bx.x & 0xFFFFu, y0 = bx.x >> 16, x1 = bx.y & 0xFFFFu, y1 = bx.y >> 16;
-          ov = bx.x != 0xFFFFFFFFu && x0 <= qx0 + 7u && x1 >= qx0 && y0 <= qy0 + 7u && y1 >= qy0;
+          ov = bx.x != 0xFFFFFFFFu && x0 <= ax1 && x1 >= ax0 && y0 <= ay1 && y1 >= ay0;
           if (ov && !SUBS) ov = range_meets(s_m[buf][j], s_t[buf][j], q_lo.x, q_hi.x, q_lo.y, q_hi.y);
         }
         if constexpr (!SUBS) {
@@ -422,7 +438,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           // the same exact test per 4x4 sub-block, one list each
 #pragma unroll
           for (int sb = 0; sb < 4; ++sb) {
-            bool o = ov;
+            // a sub-block whose pixels are all opaque takes no entries
+            bool o = ov && ((alive >> (16 * sb)) & 0xFFFFull) != 0ull;
             if (o) {
               const uint32_t bx0 = qx0 + (sb & 1) * 4u, by0 = qy0 + (sb >> 1) * 4u;
               const uint2 bx = s_box[buf][j];
@@ -552,7 +569,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           }
         }
       }
-      wave_done = __ballot(inside && tr != 0.0f) == 0ull;
+      alive = __ballot(inside && tr != 0.0f);
+      wave_done = alive == 0ull;
     }
     if (more) stage(buf ^ 1);
     buf ^= 1;
