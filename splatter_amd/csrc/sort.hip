@@ -24,6 +24,11 @@
 
 namespace spl {
 
+template <bool B>
+struct Flag {
+  static constexpr bool value = B;
+};
+
 template <int BITS, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_onesweep_pass(SortPassLaunch a) {
   constexpr int RADIX = 1 << BITS;
@@ -76,14 +81,20 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
   }
 
-  while (true) {
-    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-    for (int i = tid; i < WAVES * RADIX; i += THREADS) s_wave_hist[i] = 0u;
-    __syncthreads();
-    const uint32_t t = s_tile;
-    if (t >= num_tiles) break;
+  // the key bit that holds bit b of the digit, per b (uniform): a run-time
+  // offset keeps the extraction a single v_bfe_i32.  Its offset field has 5
+  // bits, so every pass needs shift + BITS <= 32: the depth keys' passes end
+  // at bit 31 (8-bit digits) or 29 (10-bit), the tile keys' at bit 23.
+  static_assert(BITS <= 16, "digit width");
+  uint32_t bit_at[BITS];
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) bit_at[b] = (uint32_t)shift + (uint32_t)b;
+
+  // One tile.  FULL tiles (all but the last of a view) hold TILE keys and
+  // skip every per-key bounds test.
+  auto tile_pass = [&](auto full_tag, const uint32_t t, const uint32_t tile_n) {
+    constexpr bool FULL = decltype(full_tag)::value;
     const uint32_t tile_base = t * TILE;
-    const uint32_t tile_n = (n - tile_base) < (uint32_t)TILE ? (n - tile_base) : (uint32_t)TILE;
 
     // Register budget: keys die once they are in LDS, values are fetched
     // while the look-back waits, and the scatter keeps digits packed (four
@@ -93,7 +104,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t idx = wave_base + j * kWave;
-      key[j] = idx < n ? keys_in[idx] : 0xFFFFFFFFu;
+      key[j] = FULL || idx < n ? keys_in[idx] : 0xFFFFFFFFu;
     }
 
     // WLMS ranking, items in input order.  The wave's digit counters are
@@ -104,18 +115,19 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
         (volatile __attribute__((address_space(3))) uint32_t*)(s_wave_hist + w * RADIX);
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-      const bool valid = (wave_base + j * kWave) < n;
+      const bool valid = FULL || (wave_base + j * kWave) < n;
       const uint32_t d = (key[j] >> shift) & MASK;
-      const uint64_t vm = __ballot(valid);
+      const uint64_t vm = FULL ? ~0ull : __ballot(valid);
       uint32_t plo = (uint32_t)vm, phi = (uint32_t)(vm >> 32);
 #pragma unroll
       for (int b = 0; b < BITS; ++b) {
-        // lanes whose bit b equals mine: the ballot, or its complement
-        const uint32_t bit = (d >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        const uint32_t flip = bit - 1u;  // 0 when set, all ones when clear
-        plo &= (uint32_t)bal ^ flip;
-        phi &= (uint32_t)(bal >> 32) ^ flip;
+        // lanes whose bit b equals mine: the ballot where the bit is set,
+        // its complement where clear (m is the bit sign-extended to a word;
+        // one v_bitop3 per half computes acc & ~(bal ^ m), truth table 0x90)
+        const uint32_t m = (uint32_t)__builtin_amdgcn_sbfe((int32_t)key[j], bit_at[b], 1);
+        const uint64_t bal = __ballot(m != 0u);
+        plo = __builtin_amdgcn_bitop3_b32(plo, (uint32_t)bal, m, 0x90);
+        phi = __builtin_amdgcn_bitop3_b32(phi, (uint32_t)(bal >> 32), m, 0x90);
       }
       const uint64_t peers = ((uint64_t)phi << 32) | plo;
       const uint32_t below = lanes_below(peers);
@@ -170,7 +182,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     // keys to their place in the tile, sorted by digit
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-      if (wave_base + j * kWave < n) {
+      if (FULL || wave_base + j * kWave < n) {
         const uint32_t d = (key[j] >> shift) & MASK;
         pos[j] += s_local[d] + s_wave_hist[w * RADIX + d];
         if (pos[j] < (uint32_t)TILE) s_buf[pos[j]] = key[j];
@@ -180,7 +192,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t idx = wave_base + j * kWave;
-      val[j] = idx < n ? (vals_in ? vals_in[idx] : idx) : 0u;
+      val[j] = FULL || idx < n ? (vals_in ? vals_in[idx] : idx) : 0u;
     }
     {
       uint32_t excl = excl0;
@@ -205,7 +217,7 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const uint32_t i = k * THREADS + tid;
-      if (i < tile_n) {
+      if (FULL || i < tile_n) {
         const uint32_t kk = s_buf[i];
         const uint32_t d = (kk >> shift) & MASK;
         dig[k / PER] |= d << (PW * (k % PER));
@@ -219,18 +231,29 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j)
-      if (wave_base + j * kWave < n && pos[j] < (uint32_t)TILE) s_buf[pos[j]] = val[j];
+      if ((FULL || wave_base + j * kWave < n) && pos[j] < (uint32_t)TILE) s_buf[pos[j]] = val[j];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const uint32_t i = k * THREADS + tid;
-      if (i < tile_n) {
+      if (FULL || i < tile_n) {
         const uint32_t d = (dig[k / PER] >> (PW * (k % PER))) & ((1u << PW) - 1u);
         const uint32_t o = s_base[d] + i;
         if (o < n) vals_out[o] = s_buf[i];
       }
     }
     __syncthreads();
+  };
+
+  while (true) {
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < WAVES * RADIX; i += THREADS) s_wave_hist[i] = 0u;
+    __syncthreads();
+    const uint32_t t = s_tile;
+    if (t >= num_tiles) break;
+    const uint32_t left = n - t * TILE;
+    if (left >= (uint32_t)TILE) tile_pass(Flag<true>{}, t, (uint32_t)TILE);
+    else tile_pass(Flag<false>{}, t, left);
   }
   finish_persistent(tile_counter, counters + kDone + a.tile_slot);
 }
