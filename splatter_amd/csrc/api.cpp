@@ -82,8 +82,7 @@ int spl_device_close(spl_device* d) {
   if (!d) return SPL_OK;
   (void)hipSetDevice(d->ordinal);
   (void)hipStreamSynchronize(d->stream);
-  (void)hipStreamDestroy(d->stream);
-  delete d;
+  spl::device_release(d);
   return SPL_OK;
 }
 
@@ -104,6 +103,16 @@ int spl_scene_create(spl_renderer* r, uint64_t splat_count, spl_scene** out) {
 }
 
 }  // extern "C"
+
+void spl::device_retain(spl_device* d) { d->refs.fetch_add(1); }
+
+void spl::device_release(spl_device* d) {
+  if (d->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(d->ordinal);
+  (void)hipStreamSynchronize(d->stream);
+  (void)hipStreamDestroy(d->stream);
+  delete d;
+}
 
 int spl::scene_create(spl_device* dev, uint64_t count, spl_scene** out) {
   *out = nullptr;
@@ -139,6 +148,7 @@ int spl::scene_create(spl_device* dev, uint64_t count, spl_scene** out) {
     return fail(e == hipErrorOutOfMemory ? SPL_ERROR_OUT_OF_MEMORY : SPL_ERROR_DEVICE,
                 std::string("spl_scene_create: ") + hipGetErrorString(e));
   }
+  device_retain(dev);
   *out = s;
   return SPL_OK;
 }
@@ -159,7 +169,9 @@ int spl_scene_destroy(spl_scene* s) {
   dev_free(s->rotations);
   dev_free(s->scale_alpha);
   dev_free(s->sh);
+  spl_device* d = s->dev;
   delete s;
+  spl::device_release(d);
   return SPL_OK;
 }
 
@@ -340,6 +352,7 @@ int spl_frame_create(spl_device* d, uint32_t w, uint32_t h, spl_frame** out) {
     delete f;
     return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_frame_create: device memory");
   }
+  spl::device_retain(d);
   *out = f;
   return SPL_OK;
 }
@@ -349,7 +362,9 @@ int spl_frame_destroy(spl_frame* f) {
   (void)hipSetDevice(f->dev->ordinal);
   (void)hipStreamSynchronize(f->dev->stream);
   dev_free(f->pixels);
+  spl_device* d = f->dev;
   delete f;
+  spl::device_release(d);
   return SPL_OK;
 }
 

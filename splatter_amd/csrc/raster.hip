@@ -96,7 +96,11 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
       const uint32_t r = t * TILE + slot;
       cnt[k] = 0;
       if (r < n) {
-        const uint32_t c = order ? order[r] : r;
+        uint32_t c = order ? order[r] : r;
+        if (c >= n) {  // only after an upstream error; never read past the boxes
+          counters[kScatterError] = 1u;
+          c = 0;
+        }
         const uint2 q = boxes[c];
         const uint32_t lo = q.x, hi = q.y;
         const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;

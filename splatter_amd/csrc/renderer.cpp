@@ -704,6 +704,7 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
       return rc;
     }
   }
+  device_retain(dev);
   *out = r;
   return SPL_OK;
 }
@@ -730,7 +731,9 @@ int spl_renderer_destroy(spl_renderer* r) {
   (void)hipSetDevice(r->dev->ordinal);
   (void)hipStreamSynchronize(r->stream);
   renderer_free(r);
+  spl_device* d = r->dev;
   delete r;
+  spl::device_release(d);
   return SPL_OK;
 }
 

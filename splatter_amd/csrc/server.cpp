@@ -361,6 +361,7 @@ int spl_server_create(spl_device* dev, const spl_server_configuration* cfg, spl_
   auto* s = new (std::nothrow) spl_server();
   if (!s) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_server_create: host allocation");
   s->dev = dev;
+  device_retain(dev);
   s->cfg = *cfg;
   s->presize_tiles = div_up(cfg->max_width, kTileSize) * div_up(cfg->max_height, kTileSize);
   s->t0 = std::chrono::steady_clock::now();
@@ -375,7 +376,9 @@ int spl_server_destroy(spl_server* s) {
     if (sl.busy) (void)hipEventSynchronize(sl.done);
   for (size_t i = 0; i < s->slots.size(); ++i) release_slot(s, i);
   for (size_t i = 0; i < s->frames.size(); ++i) release_frame(s, i);
+  spl_device* d = s->dev;
   delete s;
+  spl::device_release(d);
   return SPL_OK;
 }
 

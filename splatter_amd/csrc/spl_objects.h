@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 
 #include "splatter.h"
@@ -32,6 +33,12 @@ int renderer_counter_words(const spl_renderer* r, uint32_t view, const uint32_t*
 // Scene::new without a renderer: `count` splats, zeroed, on `dev`.
 int scene_create(spl_device* dev, uint64_t count, spl_scene** out);
 
+// Every renderer, scene, frame and server holds a reference to its device,
+// as does the handle spl_device_open returns: the device (and its stream)
+// goes with the last of them, whatever order they are destroyed in.
+void device_retain(spl_device* d);
+void device_release(spl_device* d);
+
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 template <typename T>
@@ -59,6 +66,7 @@ struct spl_device {
   int ordinal = 0;
   hipStream_t stream = nullptr;
   int cu_count = 0;
+  std::atomic<int> refs{1};  // the opener's handle + one per object created on the device
 };
 
 struct spl_scene {

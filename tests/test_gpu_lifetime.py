@@ -1,0 +1,55 @@
+"""Object lifetimes through the C ABI: a device handle may be closed before
+the renderers, scenes, frames and servers created on it, and those may be
+destroyed in any order (a garbage collector at interpreter exit picks its
+own order).  Each object keeps its device alive."""
+import numpy as np
+import pytest
+
+import splatter_amd as sa
+from gpu_common import config, say
+from splatter_amd.synth import Workload, load_scene, sphere_pose
+
+pytestmark = pytest.mark.gpu
+
+
+def _render(dev, w, close_device_first: bool):
+    r = sa.Renderer(dev, config(max_splat_count=w.count))
+    s = sa.Scene(r, w.count)
+    load_scene(s, w)
+    f = sa.Frame(dev, 160, 120)
+    if close_device_first:
+        dev.close()
+    cam, view = sphere_pose(11, radius=4)
+    r.render_frame(f, cam, view, s)
+    img = f.read()
+    return img, (r, s, f)
+
+
+def test_device_closed_before_its_objects():
+    w = Workload("gaussian", 20_000, seed=11)
+    ref, objs = _render(sa.Device(0), w, close_device_first=False)
+    for o in objs:
+        o.close()
+    for order in ((0, 1, 2), (2, 0, 1), (1, 2, 0)):
+        say(f"destroy order {order}")
+        img, objs = _render(sa.Device(0), w, close_device_first=True)
+        assert np.array_equal(img, ref)
+        for i in order:
+            objs[i].close()
+
+
+def test_server_outlives_its_device_handle():
+    dev = sa.Device(0)
+    cfg = sa.Configuration(spherical_harmonics_order=3, max_splat_count=20_000, max_views=2)
+    server = sa.Server(dev, cfg, max_in_flight=2, max_width=320, max_height=240, memory_budget_bytes=4 << 30)
+    scene = server.create_scene(20_000)
+    load_scene(scene, Workload("gaussian", 20_000, seed=12))
+    dev.close()
+    cam, view = sphere_pose(12, radius=4)
+    req = server.submit(scene, 160, 120, cam, view)
+    server.wait(req)
+    img = server.read(req)
+    assert img.shape == (120, 160, 4) and (img[..., 3] < 255).any()
+    server.release(req)
+    server.close()
+    scene.close()
