@@ -25,14 +25,13 @@ namespace {
 // map, so a pixel costs two fused products per coordinate and one exp2.
 constexpr float kGaussScale = 0.84932180f;
 
-// 2^max(y, -125) for y <= 0: y = k + f with k = rint(y), |f| <= 1/2, the
-// degree-6 Taylor polynomial of e^(f ln 2) in fused Horner form, then
-// ldexp by k (exact: the result stays normal).  Only correctly rounded IEEE
+// 2^y for y <= 0: y = k + f with k = rint(y), |f| <= 1/2, the degree-6
+// Taylor polynomial of e^(f ln 2) in fused Horner form, then ldexp by k
+// (exact while the result stays normal).  Only correctly rounded IEEE
 // operations (fma, rint, ldexp), so it rounds the same on the CPU and the
-// GPU.  Below -8 every alpha is already under the 1/255 fragment threshold,
-// so the clamp changes no pixel.
+// GPU.  The oracle clamps y at -125 first; below -8 every alpha is under the
+// 1/255 fragment threshold either way, so the pixels agree without it.
 __device__ __forceinline__ float spl_exp2(float y) {
-  y = fmaxf(y, -125.0f);
   const float k = rintf(y);
   const float f = y - k;
   float p = 1.54035304e-4f;
@@ -269,9 +268,9 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   static_assert(B == 128 || B == 256, "entries staged per batch");
   constexpr int U = B / 64;  // list slots per step of the entry-major walks
   constexpr bool WALK = MODE == 1, SUBS = MODE == 2;
-  __shared__ float4 s_m[2][B];     // i00 i01 i10 i11
-  __shared__ float4 s_t[2][B];     // tx ty half alpha
-  __shared__ float4 s_c[2][B];     // r g b -
+  // per staged entry: [0] i00 i01 i10 i11, [1] tx ty half alpha, [2] r g b -
+  // (one array, so an entry's three rows share one LDS address)
+  __shared__ float4 s_ent[2][3][B];
   __shared__ uint2 s_box[2][B];    // pixel box lo / hi (x | y << 16)
   __shared__ __align__(16) uint8_t s_list[4][B];  // per-wave compacted entry slots, read 4 at a time
   __shared__ __align__(16) uint8_t s_sub[SUBS ? 4 : 1][SUBS ? 4 : 1][SUBS ? B : 4];  // mode 2: per 4x4 sub-block
@@ -371,13 +370,13 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     const float A00 = g1.x, A10 = g1.y, A01 = g1.z, A11 = g1.w;
     const float det = A00 * A11 - A10 * A01;
     const float rs = (1.0f / det) * kGaussScale;  // inverse map scaled by S
-    s_m[buf][tid] = make_float4(A11 * rs, -A01 * rs, -A10 * rs, A00 * rs);
+    s_ent[buf][0][tid] = make_float4(A11 * rs, -A01 * rs, -A10 * rs, A00 * rs);
     // unaligned: the rectangle |tc| < margin in scaled coordinates;
     // axis-aligned: a square of half-side radius * margin in the view plane
     const float half = unaligned ? margin * kGaussScale
                                  : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
-    s_t[buf][tid] = make_float4(g2.x, g2.y, half, g0.w);
-    s_c[buf][tid] = make_float4(g0.x, g0.y, g0.z, 0.0f);
+    s_ent[buf][1][tid] = make_float4(g2.x, g2.y, half, g0.w);
+    s_ent[buf][2][tid] = make_float4(g0.x, g0.y, g0.z, 0.0f);
     s_box[buf][tid] = make_uint2(__float_as_uint(g2.z), __float_as_uint(g2.w));
   };
 
@@ -432,7 +431,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           const uint2 bx = s_box[buf][j];
           const uint32_t x0 = bx.x & 0xFFFFu, y0 = bx.x >> 16, x1 = bx.y & 0xFFFFu, y1 = bx.y >> 16;
           ov = bx.x != 0xFFFFFFFFu && x0 <= ax1 && x1 >= ax0 && y0 <= ay1 && y1 >= ay0;
-          if (ov && !SUBS) ov = range_meets(s_m[buf][j], s_t[buf][j], q_lo.x, q_hi.x, q_lo.y, q_hi.y);
+          if (ov && !SUBS) ov = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], q_lo.x, q_hi.x, q_lo.y, q_hi.y);
         }
         if constexpr (!SUBS) {
           const uint64_t bm = __ballot(ov);
@@ -455,7 +454,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
                 const float hix = ((((float)bx1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
                 const float loy = (1.0f - (((float)by1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
                 const float hiy = (1.0f - (((float)by0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
-                o = range_meets(s_m[buf][j], s_t[buf][j], lox, hix, loy, hiy);
+                o = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], lox, hix, loy, hiy);
               }
             }
             const uint64_t bm = __ballot(o);
@@ -476,24 +475,27 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
         return unaligned ? fmaxf(fabsf(u), fabsf(w2)) < h : fmaxf(fabsf(dx), fabsf(dy)) < h;
       };
       auto shade = [&](float u, float w2, float opacity, const float4& col) {
-        const float alpha = opacity * spl_exp2(-fmaf(u, u, w2 * w2));
-        if (alpha >= 1.0f / 255.0f) {
+        float alpha = opacity * spl_exp2(-fmaf(u, u, w2 * w2));
+        if constexpr (!f32) {
+          // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
+          // 255) and rint(T * (1 - a)): the ideal 8-bit blender.  Below the
+          // threshold alpha becomes 0, which leaves C and T as they are
+          // (they hold integers; clamp01 of a NaN product is 0), so no
+          // branch is needed.
+          alpha = alpha >= 1.0f / 255.0f ? alpha : 0.0f;
           const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
-          if constexpr (!f32) {
-            // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
-            // 255) and rint(T * (1 - a)): the ideal 8-bit blender
-            const float sa = clamp01(alpha);
-            c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
-            c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
-            c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
-            tr = rintf(fmaf(-sa, tr, tr));
-          } else {
-            const float da = tr;
-            c0 = fmaf(sr, da, c0);
-            c1 = fmaf(sg, da, c1);
-            c2 = fmaf(sb, da, c2);
-            tr = fmaf(-alpha, da, da);
-          }
+          const float sa = clamp01(alpha);
+          c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
+          c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
+          c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
+          tr = rintf(fmaf(-sa, tr, tr));
+        } else if (alpha >= 1.0f / 255.0f) {
+          const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
+          const float da = tr;
+          c0 = fmaf(sr, da, c0);
+          c1 = fmaf(sg, da, c1);
+          c2 = fmaf(sb, da, c2);
+          tr = fmaf(-alpha, da, da);
         }
       };
       if constexpr (SUBS) {
@@ -509,9 +511,9 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
 #pragma unroll
           for (int k = 0; k < U; ++k) {
             const uint32_t j = (js >> (8 * k)) & 255u;  // slots past the list hold stale, in-range indices
-            m[k] = s_m[buf][j];
-            t[k] = s_t[buf][j];
-            col[k] = s_c[buf][j];
+            m[k] = s_ent[buf][0][j];
+            t[k] = s_ent[buf][1][j];
+            col[k] = s_ent[buf][2][j];
           }
 #pragma unroll
           for (int k = 0; k < U; ++k) {
@@ -530,9 +532,9 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
 #pragma unroll
           for (int k = 0; k < U; ++k) {
             const uint32_t j = (js >> (8 * k)) & 255u;  // slots past n hold stale, in-range indices
-            m[k] = s_m[buf][j];
-            t[k] = s_t[buf][j];
-            col[k] = s_c[buf][j];
+            m[k] = s_ent[buf][0][j];
+            t[k] = s_ent[buf][1][j];
+            col[k] = s_ent[buf][2][j];
           }
 #pragma unroll
           for (int k = 0; k < U; ++k) {
@@ -554,7 +556,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
             for (int q = 0; q < 4; ++q) {
               const uint32_t j = (js >> (8 * q)) & 255u;
               float u, w2;
-              const uint32_t in = covers(s_m[buf][j], s_t[buf][j], u, w2) && k + q < cnt ? 1u : 0u;
+              const uint32_t in = covers(s_ent[buf][0][j], s_ent[buf][1][j], u, w2) && k + q < cnt ? 1u : 0u;
               if (k < 32) lo |= in << (k + q);
               else hi |= in << (k + q - 32);
             }
@@ -565,10 +567,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
             if (lo) { k = (uint32_t)__builtin_ctz(lo); lo &= lo - 1u; }
             else { k = 32u + (uint32_t)__builtin_ctz(hi); hi &= hi - 1u; }
             const uint32_t j = s_list[w][c + k];
-            const float4 t = s_t[buf][j];
+            const float4 t = s_ent[buf][1][j];
             float u, w2;
-            (void)covers(s_m[buf][j], t, u, w2);
-            shade(u, w2, t.w, s_c[buf][j]);
+            (void)covers(s_ent[buf][0][j], t, u, w2);
+            shade(u, w2, t.w, s_ent[buf][2][j]);
             if (tr == 0.0f) lo = hi = 0u;
           }
         }
