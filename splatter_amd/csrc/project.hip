@@ -12,8 +12,10 @@
 // (shaders.wgsl:217-237) lose most of their digits in FP32 to the
 // cancellation in e = M.zz - t^T M t.  Products and sums follow the WGSL's
 // order; in FP64 a product and the sum it feeds are fused (one rounding
-// instead of two, 2^-53 relative) and divisions by the same value share one
-// reciprocal, which keeps them far inside the 1e-5 conic tolerance.  The
+// instead of two, 2^-53 relative), divisions by the same value share one
+// reciprocal, and reciprocals and square roots are the hardware estimates
+// refined by one Newton step (an ulp or two), all far inside the 1e-5 conic
+// tolerance.  The
 // f32 colour path is not contracted: it matches the reference's f32
 // arithmetic operation for operation.
 #include "spl_device.h"
@@ -26,6 +28,24 @@ namespace {
 // Matrices are WGSL column-major 3x3: m[c][r].
 
 __device__ __forceinline__ double sgn(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+// 1/x: the hardware reciprocal and one Newton step (the error squares, to
+// within an ulp or two of the division), IEEE results for 0 and infinity.
+// About a third of the instructions of a correctly rounded division.
+__device__ __forceinline__ double rcp64(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, r, 1.0);
+  return e == e ? fma(r, e, r) : r;
+}
+
+// sqrt(x): the hardware reciprocal square root y, h = x y and one Newton
+// step h + (x - h^2) y / 2; 0 and infinity as IEEE, NaN below 0.
+__device__ __forceinline__ double sqrt64(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double h = x * y;
+  const double s = fma(fma(-h, h, x), 0.5 * y, h);
+  return x == 0.0 || x == __builtin_inf() ? x : s;
+}
 
 // shaders.wgsl:63-79 quatToMat
 __device__ __forceinline__ void quat_to_mat(const double p[4], double m[3][3]) {
@@ -46,7 +66,7 @@ __device__ __forceinline__ void contour_transform(const double scale[3], const d
   quat_to_mat(rot, T);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const double inv = 1.0 / scale[c];
+    const double inv = rcp64(scale[c]);
 #pragma unroll
     for (int r = 0; r < 3; ++r) T[c][r] = T[c][r] * inv;
   }
@@ -115,7 +135,7 @@ __device__ void covariance2(const double T[3][3], const double pos[3], const Uni
 #pragma unroll
   for (int r = 0; r < 3; ++r)
     vp[r] = (((double)V[r] * pos[0] + (double)V[4 + r] * pos[1]) + (double)V[8 + r] * pos[2]) + (double)V[12 + r];
-  const double iz = 1.0 / vp[2];
+  const double iz = rcp64(vp[2]);
   double qx = vp[0] * iz, qy = vp[1] * iz;
   qx = qx < -1.0 ? -1.0 : (qx > 1.0 ? 1.0 : qx);
   qy = qy < -1.0 ? -1.0 : (qy > 1.0 ? 1.0 : qy);
@@ -193,15 +213,15 @@ __device__ void pixel_box(float A00, float A10, float A01, float A11, float tx, 
     ex = (fabs((double)A00) + fabs((double)A01)) * m;
     ey = (fabs((double)A10) + fabs((double)A11)) * m;
   } else {
-    ex = sqrt(fmax((double)A00 * A00 + (double)A10 * A10, (double)A01 * A01 + (double)A11 * A11)) * m;
+    ex = sqrt64(fmax((double)A00 * A00 + (double)A10 * A10, (double)A01 * A01 + (double)A11 * A11)) * m;
     ey = ex;
   }
   const double W = (double)u.image_size[0], H = (double)u.image_size[1];
-  const double vw = (double)u.view_size[0], vh = (double)u.view_size[1];
-  const double pcx = ((double)tx / vw + 1.0) * 0.5 * W, pcy = (1.0 - (double)ty / vh) * 0.5 * H;
+  const double ivw = rcp64((double)u.view_size[0]), ivh = rcp64((double)u.view_size[1]);
+  const double pcx = ((double)tx * ivw + 1.0) * 0.5 * W, pcy = (1.0 - (double)ty * ivh) * 0.5 * H;
   // +1 px: pixel centres sit half a pixel inside the box edges, the rest
   // absorbs rounding of the f32 record and of the coverage test
-  const double rx = ex / vw * 0.5 * W + 1.0, ry = ey / vh * 0.5 * H + 1.0;
+  const double rx = ex * ivw * 0.5 * W + 1.0, ry = ey * ivh * 0.5 * H + 1.0;
   const double fx0 = floor(pcx - rx), fx1 = ceil(pcx + rx);
   const double fy0 = floor(pcy - ry), fy1 = ceil(pcy + ry);
   if (fx1 >= 0.0 && fy1 >= 0.0 && fx0 <= W - 1.0 && fy0 <= H - 1.0) {
@@ -294,13 +314,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         contour(Tc, pos, u, M);
         // extractTranslationOfEllipse (shaders.wgsl:179-191)
         const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
-        const double inv_disc = 1.0 / disc;
+        const double inv_disc = rcp64(disc);
         const double t0 = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * inv_disc;
         const double t1 = (M[0][1] * M[0][2] - M[0][0] * M[1][2]) * inv_disc;
         // extractRotationOfEllipse (shaders.wgsl:193-215)
         const double dxy = M[0][0] - M[1][1];
         const double ra = dxy * dxy;
         const double rb = ra + (4.0 * M[0][1]) * M[0][1];
+        // IEEE division and square roots here: ra / rb is exactly 1 for an
+        // axis-aligned ellipse, so rc = 1/2 and jj = 0 as in the reference
         const double rc = 0.5 * sqrt(ra / rb);
         double jj = sqrt(0.5 - rc);
         double kk = (-sqrt(0.5 + rc) * sgn(M[0][1])) * sgn(dxy);
@@ -313,15 +335,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           covariance_transform(scale, rot, Tv);
           covariance2(Tv, pos, u, c00, c11, c01);
           const double cd = c00 - c11;
+          // IEEE here: sqrt(cd * cd) is exactly |cd|, so a flat splat keeps s1 = 0
           const double bb = sqrt(cd * cd + (4.0 * c01) * c01);
-          s0 = sqrt(((c00 + c11) + bb) * 0.5);
-          s1 = sqrt(((c00 + c11) - bb) * 0.5);
+          s0 = sqrt64(((c00 + c11) + bb) * 0.5);
+          s1 = sqrt64(((c00 + c11) - bb) * 0.5);
         } else {
           // extractScaleOfEllipse (shaders.wgsl:217-237)
           const double d = ((2.0 * M[0][1]) * jj) * kk;
           const double e = M[2][2] - (((M[0][0] * t0) * t0 + (M[1][1] * t1) * t1) + ((2.0 * M[0][1]) * t0) * t1);
-          s0 = sqrt(fabs(e / (((M[0][0] * kk) * kk + (M[1][1] * jj) * jj) - d)));
-          s1 = sqrt(fabs(e / (((M[0][0] * jj) * jj + (M[1][1] * kk) * kk) + d)));
+          s0 = sqrt64(fabs(e * rcp64(((M[0][0] * kk) * kk + (M[1][1] * jj) * jj) - d)));
+          s1 = sqrt64(fabs(e * rcp64(((M[0][0] * jj) * jj + (M[1][1] * kk) * kk) + d)));
         }
         const double bias = (double)u.ellipse_size_bias;
         const double sa0 = bias + s0, sb0 = bias + s1;
