@@ -251,11 +251,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // a pair packs into 16 bits: slot in the tile << 4 | view - first view.
   constexpr int CAP = THREADS * 16;
   static_assert(TILE <= 4096 && kMaxViews <= 32, "pair packing");
+  static_assert(16 * WAVES == kWave, "one wave scans the (view, wave) counts");
   __shared__ uint32_t s_off[kMaxViews][ITEMS * WAVES];
   __shared__ uint32_t s_tp[kMaxViews];
   __shared__ uint16_t s_item[CAP];    // slot in tile << 4 | view - first view
   __shared__ uint32_t s_entry[CAP];   // entry c of that view
   __shared__ uint32_t s_tmp[WAVES];
+  // A round's records in view-major order (rank r = pairs of earlier views
+  // in the round + earlier pairs of the same view), with their destination:
+  // within one view the entries of consecutive listed splats are
+  // consecutive, so the records leave in contiguous runs.
+  __shared__ float4 s_stage[THREADS][3];
+  __shared__ uint2 s_dst[THREADS];      // view, entry
+  __shared__ uint32_t s_wcnt[16][WAVES];  // pairs per (view, wave), then their view-major offsets
   const uint32_t nv = a.views;
   for (uint32_t v = 0; v < nv; ++v) {
     if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
@@ -268,100 +276,149 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   constexpr int SH_F4 = (SH_FLOATS + 3) / 4;
   const double splat_scale = (double)a.uniforms[0].splat_scale;
 
-  // projects listed pairs [0, count) of tile t, views from v0
+  if (tid < 16 * WAVES) s_wcnt[tid / WAVES][tid % WAVES] = 0u;  // zero between rounds from here on
+
+  // Projects listed pairs [0, count) of tile t, views from v0, one round of
+  // THREADS pairs at a time.  Lanes take the pairs in list order (a splat's
+  // views side by side, so they read its attributes from the same cache
+  // lines); the records go through LDS in view-major order and leave as
+  // contiguous 16-byte chunks.  Every thread runs every round (barriers).
   auto project_pairs = [&](uint32_t t, uint32_t v0, uint32_t count) {
-    for (uint32_t k = tid; k < count; k += THREADS) {
-      const uint32_t it = s_item[k];
-      const uint32_t v = v0 + (it & 15u);
-      const uint32_t i = t * TILE + (it >> 4);
-      const uint32_t c = s_entry[k];
-      const Uniforms& u = a.uniforms[v];
-
-      const float cx = a.scene.centers[3 * (size_t)i + 0];
-      const float cy = a.scene.centers[3 * (size_t)i + 1];
-      const float cz = a.scene.centers[3 * (size_t)i + 2];
-      const float4 rq = a.scene.rotations[i];
-      const float4 sa = a.scene.scale_alpha[i];
-      float sh[SH_F4 * 4];
-      const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
+    for (uint32_t k0 = 0; k0 < count; k0 += THREADS) {
+      const uint32_t k = k0 + tid;
+      const bool valid = k < count;
+      const uint32_t it = valid ? (uint32_t)s_item[k] : 0u;
+      const uint32_t h = it & 15u;
+      // lanes of this wave holding the same view (four ballots), then the
+      // view-major rank through per-(view, wave) counts
+      uint64_t peers = __ballot(valid);
 #pragma unroll
-      for (int q = 0; q < SH_F4; ++q) {
-        const float4 x = shp[q];
-        sh[4 * q + 0] = x.x; sh[4 * q + 1] = x.y; sh[4 * q + 2] = x.z; sh[4 * q + 3] = x.w;
+      for (int b = 0; b < 4; ++b) {
+        const uint64_t bal = __ballot((h >> b) & 1u);
+        peers &= ((h >> b) & 1u) ? bal : ~bal;
       }
-
-      // shaders.wgsl:531-532: colour from the normalised ray direction
-      float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
-      const float len = sqrtf((rd[0] * rd[0] + rd[1] * rd[1]) + rd[2] * rd[2]);
-      rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
-      float rgb[3];
-      sh_color<L>(rd, sh, rgb);
-      float4* rec = a.records + v * a.rec_stride + 3 * (size_t)c;
-      rec[0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
-      // the colour's registers are free before the FP64 part starts
-      __builtin_amdgcn_sched_barrier(0);
-
-      // shaders.wgsl:533-547
-      const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
-      const double pos[3] = {(double)cx, (double)cy, (double)cz};
-      const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
-      double M[3][3];
-      float A00, A10, A01, A11, tx, ty;
-      {
-#pragma clang fp contract(fast)
-        double Tc[3][3];
-        contour_transform(scale, rot, Tc);
-        contour(Tc, pos, u, M);
-        // extractTranslationOfEllipse (shaders.wgsl:179-191)
-        const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
-        const double inv_disc = rcp64(disc);
-        const double t0 = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * inv_disc;
-        const double t1 = (M[0][1] * M[0][2] - M[0][0] * M[1][2]) * inv_disc;
-        // extractRotationOfEllipse (shaders.wgsl:193-215)
-        const double dxy = M[0][0] - M[1][1];
-        const double ra = dxy * dxy;
-        const double rb = ra + (4.0 * M[0][1]) * M[0][1];
-        // IEEE division and square roots here: ra / rb is exactly 1 for an
-        // axis-aligned ellipse, so rc = 1/2 and jj = 0 as in the reference
-        const double rc = 0.5 * sqrt(ra / rb);
-        double jj = sqrt(0.5 - rc);
-        double kk = (-sqrt(0.5 + rc) * sgn(M[0][1])) * sgn(dxy);
-        if (M[0][1] < 0.0 || dxy < 0.0) { kk = -kk; jj = -jj; }
-        if (dxy < 0.0) { const double tt = jj; jj = -kk; kk = tt; }
-        double s0, s1;
-        if (a.use_covariance_for_scale) {
-          // extractScaleOfCovariance (shaders.wgsl:240-246)
-          double Tv[3][3], c00, c11, c01;
-          covariance_transform(scale, rot, Tv);
-          covariance2(Tv, pos, u, c00, c11, c01);
-          const double cd = c00 - c11;
-          // IEEE here: sqrt(cd * cd) is exactly |cd|, so a flat splat keeps s1 = 0
-          const double bb = sqrt(cd * cd + (4.0 * c01) * c01);
-          s0 = sqrt64(((c00 + c11) + bb) * 0.5);
-          s1 = sqrt64(((c00 + c11) - bb) * 0.5);
-        } else {
-          // extractScaleOfEllipse (shaders.wgsl:217-237)
-          const double d = ((2.0 * M[0][1]) * jj) * kk;
-          const double e = M[2][2] - (((M[0][0] * t0) * t0 + (M[1][1] * t1) * t1) + ((2.0 * M[0][1]) * t0) * t1);
-          s0 = sqrt64(fabs(e * rcp64(((M[0][0] * kk) * kk + (M[1][1] * jj) * jj) - d)));
-          s1 = sqrt64(fabs(e * rcp64(((M[0][0] * jj) * jj + (M[1][1] * kk) * kk) + d)));
+      const uint32_t below = lanes_below(peers);
+      if (valid && below == 0) s_wcnt[h][w] = (uint32_t)__popcll(peers);
+      __syncthreads();
+      if (tid < kWave) {  // 16 views x 4 waves, view-major: lane = view * 4 + wave
+        const uint32_t x = s_wcnt[tid / WAVES][tid % WAVES];
+        uint32_t incl = x;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+          const uint32_t y = __shfl_up(incl, o, kWave);
+          if (tid >= (uint32_t)o) incl += y;
         }
-        const double bias = (double)u.ellipse_size_bias;
-        const double sa0 = bias + s0, sb0 = bias + s1;
-        A00 = (float)(kk * sa0); A10 = (float)(-jj * sa0);
-        A01 = (float)(jj * sb0); A11 = (float)(kk * sb0);
-        tx = (float)t0; ty = (float)t1;
+        s_wcnt[tid / WAVES][tid % WAVES] = incl - x;
       }
-      uint32_t box_lo, box_hi;
-      pixel_box(A00, A10, A01, A11, tx, ty, u, a.use_unaligned_rectangles != 0, box_lo, box_hi);
-      rec[1] = make_float4(A00, A10, A01, A11);
-      rec[2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
-      a.boxes[v * a.box_stride + c] = make_uint2(box_lo, box_hi);
-      if (a.conics && v == 0) {
-        double* cn = a.conics + 6 * (size_t)c;
-        cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];
-        cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
+      __syncthreads();
+      const uint32_t r = s_wcnt[h][w] + below;
+      if (valid) {
+        const uint32_t v = v0 + h;
+        const uint32_t i = t * TILE + (it >> 4);
+        const uint32_t c = s_entry[k];
+        const Uniforms& u = a.uniforms[v];
+
+        const float cx = a.scene.centers[3 * (size_t)i + 0];
+        const float cy = a.scene.centers[3 * (size_t)i + 1];
+        const float cz = a.scene.centers[3 * (size_t)i + 2];
+        const float4 rq = a.scene.rotations[i];
+        const float4 sa = a.scene.scale_alpha[i];
+        float sh[SH_F4 * 4];
+        const float4* shp = reinterpret_cast<const float4*>(a.scene.sh + 48 * (size_t)i);
+#pragma unroll
+        for (int q = 0; q < SH_F4; ++q) {
+          const float4 x = shp[q];
+          sh[4 * q + 0] = x.x; sh[4 * q + 1] = x.y; sh[4 * q + 2] = x.z; sh[4 * q + 3] = x.w;
+        }
+
+        // shaders.wgsl:531-532: colour from the normalised ray direction
+        float rd[3] = {cx - u.camera_matrix[12], cy - u.camera_matrix[13], cz - u.camera_matrix[14]};
+        const float len = sqrtf((rd[0] * rd[0] + rd[1] * rd[1]) + rd[2] * rd[2]);
+        rd[0] = rd[0] / len; rd[1] = rd[1] / len; rd[2] = rd[2] / len;
+        float rgb[3];
+        sh_color<L>(rd, sh, rgb);
+        s_stage[r][0] = make_float4(rgb[0], rgb[1], rgb[2], sa.w);
+        s_dst[r] = make_uint2(v, c);
+        // the colour's registers are free before the FP64 part starts
+        __builtin_amdgcn_sched_barrier(0);
+
+        // shaders.wgsl:533-547
+        const double rot[4] = {(double)rq.x, (double)rq.y, (double)rq.z, (double)rq.w};
+        const double pos[3] = {(double)cx, (double)cy, (double)cz};
+        const double scale[3] = {(double)sa.x * splat_scale, (double)sa.y * splat_scale, (double)sa.z * splat_scale};
+        double M[3][3];
+        float A00, A10, A01, A11, tx, ty;
+        {
+#pragma clang fp contract(fast)
+          double Tc[3][3];
+          contour_transform(scale, rot, Tc);
+          contour(Tc, pos, u, M);
+          // extractTranslationOfEllipse (shaders.wgsl:179-191)
+          const double disc = M[0][0] * M[1][1] - M[0][1] * M[0][1];
+          const double inv_disc = rcp64(disc);
+          const double t0 = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * inv_disc;
+          const double t1 = (M[0][1] * M[0][2] - M[0][0] * M[1][2]) * inv_disc;
+          // extractRotationOfEllipse (shaders.wgsl:193-215)
+          const double dxy = M[0][0] - M[1][1];
+          const double ra = dxy * dxy;
+          const double rb = ra + (4.0 * M[0][1]) * M[0][1];
+          // IEEE division and square roots here: ra / rb is exactly 1 for an
+          // axis-aligned ellipse, so rc = 1/2 and jj = 0 as in the reference
+          const double rc = 0.5 * sqrt(ra / rb);
+          double jj = sqrt(0.5 - rc);
+          double kk = (-sqrt(0.5 + rc) * sgn(M[0][1])) * sgn(dxy);
+          if (M[0][1] < 0.0 || dxy < 0.0) { kk = -kk; jj = -jj; }
+          if (dxy < 0.0) { const double tt = jj; jj = -kk; kk = tt; }
+          double s0, s1;
+          if (a.use_covariance_for_scale) {
+            // extractScaleOfCovariance (shaders.wgsl:240-246)
+            double Tv[3][3], c00, c11, c01;
+            covariance_transform(scale, rot, Tv);
+            covariance2(Tv, pos, u, c00, c11, c01);
+            const double cd = c00 - c11;
+            // IEEE here: sqrt(cd * cd) is exactly |cd|, so a flat splat keeps s1 = 0
+            const double bb = sqrt(cd * cd + (4.0 * c01) * c01);
+            s0 = sqrt64(((c00 + c11) + bb) * 0.5);
+            s1 = sqrt64(((c00 + c11) - bb) * 0.5);
+          } else {
+            // extractScaleOfEllipse (shaders.wgsl:217-237)
+            const double d = ((2.0 * M[0][1]) * jj) * kk;
+            const double e = M[2][2] - (((M[0][0] * t0) * t0 + (M[1][1] * t1) * t1) + ((2.0 * M[0][1]) * t0) * t1);
+            s0 = sqrt64(fabs(e * rcp64(((M[0][0] * kk) * kk + (M[1][1] * jj) * jj) - d)));
+            s1 = sqrt64(fabs(e * rcp64(((M[0][0] * jj) * jj + (M[1][1] * kk) * kk) + d)));
+          }
+          const double bias = (double)u.ellipse_size_bias;
+          const double sa0 = bias + s0, sb0 = bias + s1;
+          A00 = (float)(kk * sa0); A10 = (float)(-jj * sa0);
+          A01 = (float)(jj * sb0); A11 = (float)(kk * sb0);
+          tx = (float)t0; ty = (float)t1;
+        }
+        uint32_t box_lo, box_hi;
+        pixel_box(A00, A10, A01, A11, tx, ty, u, a.use_unaligned_rectangles != 0, box_lo, box_hi);
+        s_stage[r][1] = make_float4(A00, A10, A01, A11);
+        s_stage[r][2] = make_float4(tx, ty, __uint_as_float(box_lo), __uint_as_float(box_hi));
+        if (a.conics && v == 0) {
+          double* cn = a.conics + 6 * (size_t)c;
+          cn[0] = M[0][0]; cn[1] = M[1][1]; cn[2] = M[2][2];
+          cn[3] = M[0][1]; cn[4] = M[0][2]; cn[5] = M[1][2];
+        }
       }
+      __syncthreads();
+      // out in rank order: consecutive lanes write consecutive 16-byte
+      // chunks (and boxes) of the same view's records
+      const uint32_t n = count - k0 < (uint32_t)THREADS ? count - k0 : (uint32_t)THREADS;
+      for (uint32_t q = tid; q < 3 * n; q += THREADS) {
+        const uint32_t rr = q / 3, part = q - 3 * rr;
+        const uint2 d = s_dst[rr];
+        a.records[d.x * a.rec_stride + 3 * (size_t)d.y + part] = s_stage[rr][part];
+      }
+      if (tid < n) {
+        const uint2 d = s_dst[tid];
+        const float4 x = s_stage[tid][2];
+        a.boxes[d.x * a.box_stride + d.y] = make_uint2(__float_as_uint(x.z), __float_as_uint(x.w));
+      }
+      if (tid < 16 * WAVES) s_wcnt[tid / WAVES][tid % WAVES] = 0u;
+      __syncthreads();
     }
   };
 
