@@ -287,12 +287,13 @@ float orc_exp2(float y) {
     y = fmaxf(y, -125.0f);
     float k = rintf(y);
     float f = y - k;
-    float p = 1.54035304e-4f;
-    p = fmaf(p, f, 1.33335581e-3f);
-    p = fmaf(p, f, 9.61812911e-3f);
-    p = fmaf(p, f, 5.55041087e-2f);
-    p = fmaf(p, f, 2.40226507e-1f);
-    p = fmaf(p, f, 6.93147181e-1f);
+    /* relative minimax polynomial of 2^f on [-1/2, 1/2], degree 5, constant
+     * term 1 (exact at integers); within 3 ulp of libm exp2 over [-125, 0] */
+    float p = 1.32764713e-3f;
+    p = fmaf(p, f, 9.67554189e-3f);
+    p = fmaf(p, f, 5.55071309e-2f);
+    p = fmaf(p, f, 2.40221202e-1f);
+    p = fmaf(p, f, 6.93146944e-1f);
     p = fmaf(p, f, 1.0f);
     return ldexpf(p, (int)k);
 }
@@ -307,13 +308,31 @@ typedef struct {
     float tx, ty;             /* translation */
     float half;               /* coverage half-extent: margin (unaligned) or radius*margin */
     float r, g, b, a;
+    /* The same map in integer pixel coordinates (px, py): the pixel centre
+     * is q = (sx px + ox, sy py + oy), so with e = o - t
+     *   u = au px + bu py + cu,  v = av px + bv py + cv   (S * texture coordinate)
+     *   dx = sx px + ex,         dy = sy py + ey          (q - t, view plane) */
+    float au, bu, cu, av, bv, cv;
+    float sx, sy, ex, ey;
     int valid;
 } orc_prepared;
+
+/* Viewport map of pixel centres: q = (sx px + ox, sy py + oy) (inverse of the
+ * viewport transform of clip positions (T*(tc,1))/view_size, shaders.wgsl:561). */
+static void pixel_map(const orc_uniforms* u, float* sx, float* ox, float* sy, float* oy) {
+    float W = (float)u->image_size[0], H = (float)u->image_size[1];
+    float vw = u->view_size[0], vh = u->view_size[1];
+    *sx = (2.0f * vw) / W;
+    *ox = vw / W - vw;
+    *sy = -(2.0f * vh) / H;
+    *oy = vh - vh / H;
+}
 
 /* shaders.wgsl:554-571: the rectangle's inverse map.  The fragment's
  * interpolated texture coordinate equals inverse(A) * (q - t) because the
  * vertex positions are an affine image of the texture coordinates. */
-static void prepare(const orc_record* rec, float margin, int unaligned, orc_prepared* p) {
+static void prepare(const orc_record* rec, const orc_uniforms* u, int unaligned, orc_prepared* p) {
+    float margin = u->ellipse_margin;
     float A00 = rec->affine[0], A10 = rec->affine[1], A01 = rec->affine[2], A11 = rec->affine[3];
     p->valid = 1;
     for (int i = 0; i < 6; ++i)
@@ -333,22 +352,34 @@ static void prepare(const orc_record* rec, float margin, int unaligned, orc_prep
         p->half = sqrtf(fmaxf(d0, d1)) * margin;
     }
     p->r = rec->color[0]; p->g = rec->color[1]; p->b = rec->color[2]; p->a = rec->color[3];
+    float ox, oy;
+    pixel_map(u, &p->sx, &ox, &p->sy, &oy);
+    p->ex = ox - p->tx;
+    p->ey = oy - p->ty;
+    p->au = p->i00 * p->sx;
+    p->bu = p->i01 * p->sy;
+    p->cu = fmaf(p->i00, p->ex, p->i01 * p->ey);
+    p->av = p->i10 * p->sx;
+    p->bv = p->i11 * p->sy;
+    p->cv = fmaf(p->i10, p->ex, p->i11 * p->ey);
 }
-
-/* Pixel centre -> view-plane coordinate (inverse of the viewport transform
- * of clip positions (T*(tc,1))/view_size, shaders.wgsl:561). */
-static inline float pixel_qx(int px, float w, float vw) { return ((((float)px + 0.5f) / w) * 2.0f - 1.0f) * vw; }
-static inline float pixel_qy(int py, float h, float vh) { return (1.0f - (((float)py + 0.5f) / h) * 2.0f) * vh; }
 
 /* One fragment: coverage, alpha (shaders.wgsl:585-589) and blend
  * (renderer.rs:260-271: colour = src*DstAlpha + dst, alpha = dst*(1-srcA)). */
-static inline void shade(const orc_prepared* p, int unaligned, float qx, float qy,
+static inline void shade(const orc_prepared* p, int unaligned, float px, float py,
                          int blend_mode, float* st) {
-    float dx = qx - p->tx, dy = qy - p->ty;
-    float u = fmaf(p->i00, dx, p->i01 * dy); /* S * texture coordinate */
-    float v = fmaf(p->i10, dx, p->i11 * dy);
-    float h = p->half;
-    int inside = unaligned ? fmaxf(fabsf(u), fabsf(v)) < h : fmaxf(fabsf(dx), fabsf(dy)) < h;
+    float u, v, h = p->half;
+    int inside;
+    if (unaligned) {
+        u = fmaf(p->au, px, fmaf(p->bu, py, p->cu)); /* S * texture coordinate */
+        v = fmaf(p->av, px, fmaf(p->bv, py, p->cv));
+        inside = fmaxf(fabsf(u), fabsf(v)) < h;
+    } else {
+        float dx = fmaf(p->sx, px, p->ex), dy = fmaf(p->sy, py, p->ey);
+        u = fmaf(p->i00, dx, p->i01 * dy);
+        v = fmaf(p->i10, dx, p->i11 * dy);
+        inside = fmaxf(fabsf(dx), fabsf(dy)) < h;
+    }
     if (!inside) return;
     float alpha = p->a * orc_exp2(-fmaf(u, u, v * v));
     if (!(alpha >= 1.0f / 255.0f)) return;
@@ -386,7 +417,7 @@ void orc_raster(const orc_record* records, size_t count, const orc_uniforms* u,
     }
     for (size_t s = 0; s < count; ++s) {
         orc_prepared p;
-        prepare(&records[s], margin, unaligned, &p);
+        prepare(&records[s], u, unaligned, &p);
         if (!p.valid) continue;
         int x0 = 0, x1 = W - 1, y0 = 0, y1 = H - 1;
         if (!brute_force) {
@@ -408,12 +439,11 @@ void orc_raster(const orc_record* records, size_t count, const orc_uniforms* u,
             y1 = fy1 > H - 1 ? H - 1 : (int)fy1;
         }
         for (int py = y0; py <= y1; ++py) {
-            float qy = pixel_qy(py, (float)H, vh);
             for (int px = x0; px <= x1; ++px) {
                 float* st = &state[4 * ((size_t)py * W + px)];
                 /* transmittance 0: every later fragment adds src * 0 */
                 if (st[3] == 0.0f) continue;
-                shade(&p, unaligned, pixel_qx(px, (float)W, vw), qy, blend_mode, st);
+                shade(&p, unaligned, (float)px, (float)py, blend_mode, st);
             }
         }
     }

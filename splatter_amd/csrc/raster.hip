@@ -25,8 +25,9 @@ namespace {
 // map, so a pixel costs two fused products per coordinate and one exp2.
 constexpr float kGaussScale = 0.84932180f;
 
-// 2^y for y <= 0: y = k + f with k = rint(y), |f| <= 1/2, the degree-6
-// Taylor polynomial of e^(f ln 2) in fused Horner form, then ldexp by k
+// 2^y for y <= 0: y = k + f with k = rint(y), |f| <= 1/2, a degree-5
+// relative minimax polynomial of 2^f (constant term 1, so exact at
+// integers; within 3 ulp of exp2) in fused Horner form, then ldexp by k
 // (exact while the result stays normal).  Only correctly rounded IEEE
 // operations (fma, rint, ldexp), so it rounds the same on the CPU and the
 // GPU.  The oracle clamps y at -125 first; below -8 every alpha is under the
@@ -34,12 +35,11 @@ constexpr float kGaussScale = 0.84932180f;
 __device__ __forceinline__ float spl_exp2(float y) {
   const float k = rintf(y);
   const float f = y - k;
-  float p = 1.54035304e-4f;
-  p = fmaf(p, f, 1.33335581e-3f);
-  p = fmaf(p, f, 9.61812911e-3f);
-  p = fmaf(p, f, 5.55041087e-2f);
-  p = fmaf(p, f, 2.40226507e-1f);
-  p = fmaf(p, f, 6.93147181e-1f);
+  float p = 1.32764713e-3f;
+  p = fmaf(p, f, 9.67554189e-3f);
+  p = fmaf(p, f, 5.55071309e-2f);
+  p = fmaf(p, f, 2.40221202e-1f);
+  p = fmaf(p, f, 6.93146944e-1f);
   p = fmaf(p, f, 1.0f);
   return ldexpf(p, (int)k);
 }
@@ -326,29 +326,40 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
   __syncthreads();
   if (tid == 0) ranges[tile] = make_uint2(0u, 0u);
 
-  const float qx = ((((float)px + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
-  const float qy = (1.0f - (((float)py + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
+  // Pixel centres in the view plane are q = (sx px + ox, sy py + oy)
+  // (shaders.wgsl:561 inverted).  Each staged entry folds this map into its
+  // own: with e = o - t the coverage coordinate is affine in the integer
+  // pixel coordinates,
+  //   unaligned:     u = au px + bu py + cu,  w = av px + bv py + cv
+  //   axis-aligned:  dx = sx px + ex,  dy = sy py + ey  (then u, w from dx, dy)
+  // so a pixel costs two fused products per coordinate.
+  const float Wf = (float)W, Hf = (float)H, vw = u.view_size[0], vh = u.view_size[1];
+  const float sx = (2.0f * vw) / Wf, ox = vw / Wf - vw;
+  const float sy = -(2.0f * vh) / Hf, oy = vh - vh / Hf;
+  const float pxf = (float)px, pyf = (float)py;
   constexpr bool f32 = F32;
   constexpr bool unaligned = UNALIGNED;
-  // Can entry (m, t) cover a pixel centre of the box [lox, hix] x [loy, hiy]
-  // (view plane)?  The tested coordinate pair is affine in the centre, so its
-  // exact range over the box sits at the corners; the slack is far above the
-  // rounding of the per-pixel evaluation.
-  auto range_meets = [&](const float4& m, const float4& t, float lox, float hix, float loy, float hiy) -> bool {
-    const float dxl = lox - t.x, dxh = hix - t.x, dyl = loy - t.y, dyh = hiy - t.y;
-    float ax0, ax1, ay0, ay1;  // range of the tested coordinate pair
-    if constexpr (unaligned) {
-      ax0 = fminf(m.x * dxl, m.x * dxh) + fminf(m.y * dyl, m.y * dyh);
-      ax1 = fmaxf(m.x * dxl, m.x * dxh) + fmaxf(m.y * dyl, m.y * dyh);
-      ay0 = fminf(m.z * dxl, m.z * dxh) + fminf(m.w * dyl, m.w * dyh);
-      ay1 = fmaxf(m.z * dxl, m.z * dxh) + fmaxf(m.w * dyl, m.w * dyh);
-    } else {
-      ax0 = dxl; ax1 = dxh; ay0 = dyl; ay1 = dyh;
-    }
+  // Can entry (m, t) cover a pixel centre of the pixel box [x0, x1] x
+  // [y0, y1]?  The tested coordinates are affine in the pixel, so their exact
+  // range over the box sits at the corners; the slack (relative to the
+  // magnitudes the per-pixel evaluation adds up) is far above its rounding.
+  auto range_meets = [&](const float4& m, const float4& t, float x0, float x1, float y0, float y1) -> bool {
     const float h = t.z;
-    const float sx = 1e-4f * fmaxf(fmaxf(fabsf(ax0), fabsf(ax1)), h) + 1e-30f;
-    const float sy = 1e-4f * fmaxf(fmaxf(fabsf(ay0), fabsf(ay1)), h) + 1e-30f;
-    return ax1 + sx >= -h && ax0 - sx < h && ay1 + sy >= -h && ay0 - sy < h;
+    float a0, a1, b0, b1, sa, sb;  // ranges of the tested coordinates, and slack
+    if constexpr (unaligned) {
+      a0 = (fminf(m.x * x0, m.x * x1) + fminf(m.y * y0, m.y * y1)) + t.x;
+      a1 = (fmaxf(m.x * x0, m.x * x1) + fmaxf(m.y * y0, m.y * y1)) + t.x;
+      b0 = (fminf(m.z * x0, m.z * x1) + fminf(m.w * y0, m.w * y1)) + t.y;
+      b1 = (fmaxf(m.z * x0, m.z * x1) + fmaxf(m.w * y0, m.w * y1)) + t.y;
+      sa = 1e-4f * (((fabsf(m.x) * x1 + fabsf(m.y) * y1) + fabsf(t.x)) + h);
+      sb = 1e-4f * (((fabsf(m.z) * x1 + fabsf(m.w) * y1) + fabsf(t.y)) + h);
+    } else {
+      a0 = fmaf(sx, x0, t.x); a1 = fmaf(sx, x1, t.x);  // sx > 0
+      b0 = fmaf(sy, y1, t.y); b1 = fmaf(sy, y0, t.y);  // sy < 0
+      sa = 1e-4f * ((sx * x1 + fabsf(t.x)) + h);
+      sb = 1e-4f * ((fabsf(sy) * y1 + fabsf(t.y)) + h);
+    }
+    return a1 + sa >= -h && a0 - sa < h && b1 + sb >= -h && b0 - sb < h;
   };
   const float margin = u.ellipse_margin;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = f32 ? 1.0f : 255.0f;
@@ -370,12 +381,19 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
     const float A00 = g1.x, A10 = g1.y, A01 = g1.z, A11 = g1.w;
     const float det = A00 * A11 - A10 * A01;
     const float rs = (1.0f / det) * kGaussScale;  // inverse map scaled by S
-    s_ent[buf][0][tid] = make_float4(A11 * rs, -A01 * rs, -A10 * rs, A00 * rs);
+    const float i00 = A11 * rs, i01 = -A01 * rs, i10 = -A10 * rs, i11 = A00 * rs;
+    const float ex = ox - g2.x, ey = oy - g2.y;
     // unaligned: the rectangle |tc| < margin in scaled coordinates;
     // axis-aligned: a square of half-side radius * margin in the view plane
     const float half = unaligned ? margin * kGaussScale
                                  : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
-    s_ent[buf][1][tid] = make_float4(g2.x, g2.y, half, g0.w);
+    if constexpr (unaligned) {
+      s_ent[buf][0][tid] = make_float4(i00 * sx, i01 * sy, i10 * sx, i11 * sy);
+      s_ent[buf][1][tid] = make_float4(fmaf(i00, ex, i01 * ey), fmaf(i10, ex, i11 * ey), half, g0.w);
+    } else {
+      s_ent[buf][0][tid] = make_float4(i00, i01, i10, i11);
+      s_ent[buf][1][tid] = make_float4(ex, ey, half, g0.w);
+    }
     s_ent[buf][2][tid] = make_float4(g0.x, g0.y, g0.z, 0.0f);
     s_box[buf][tid] = make_uint2(__float_as_uint(g2.z), __float_as_uint(g2.w));
   };
@@ -417,10 +435,6 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
         ay0 = qy0 + ((uint32_t)__builtin_ctzll(alive) >> 3);
         ay1 = qy0 + ((63u - (uint32_t)__builtin_clzll(alive)) >> 3);
       }
-      const float2 q_lo = make_float2(((((float)ax0 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
-                                      (1.0f - (((float)ay1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
-      const float2 q_hi = make_float2(((((float)ax1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0],
-                                      (1.0f - (((float)ay0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1]);
       uint32_t n = 0;
       uint32_t ns[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -431,7 +445,8 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           const uint2 bx = s_box[buf][j];
           const uint32_t x0 = bx.x & 0xFFFFu, y0 = bx.x >> 16, x1 = bx.y & 0xFFFFu, y1 = bx.y >> 16;
           ov = bx.x != 0xFFFFFFFFu && x0 <= ax1 && x1 >= ax0 && y0 <= ay1 && y1 >= ay0;
-          if (ov && !SUBS) ov = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], q_lo.x, q_hi.x, q_lo.y, q_hi.y);
+          if (ov && !SUBS)
+            ov = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], (float)ax0, (float)ax1, (float)ay0, (float)ay1);
         }
         if constexpr (!SUBS) {
           const uint64_t bm = __ballot(ov);
@@ -450,11 +465,7 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
                   (bx.x >> 16) <= by0 + 3u && (bx.y >> 16) >= by0;
               if (o) {
                 const uint32_t bx1 = min(bx0 + 3u, W - 1u), by1 = min(by0 + 3u, H - 1u);
-                const float lox = ((((float)bx0 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
-                const float hix = ((((float)bx1 + 0.5f) / (float)W) * 2.0f - 1.0f) * u.view_size[0];
-                const float loy = (1.0f - (((float)by1 + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
-                const float hiy = (1.0f - (((float)by0 + 0.5f) / (float)H) * 2.0f) * u.view_size[1];
-                o = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], lox, hix, loy, hiy);
+                o = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], (float)bx0, (float)bx1, (float)by0, (float)by1);
               }
             }
             const uint64_t bm = __ballot(o);
@@ -468,11 +479,17 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
       // (shaders.wgsl:585-589) and the blend; identical arithmetic on both
       // walks below, so they give identical pixels
       auto covers = [&](const float4& m, const float4& t, float& u, float& w2) -> bool {
-        const float dx = qx - t.x, dy = qy - t.y;
-        u = fmaf(m.x, dx, m.y * dy);  // S * texture coordinate
-        w2 = fmaf(m.z, dx, m.w * dy);
         const float h = t.z;
-        return unaligned ? fmaxf(fabsf(u), fabsf(w2)) < h : fmaxf(fabsf(dx), fabsf(dy)) < h;
+        if constexpr (unaligned) {
+          u = fmaf(m.x, pxf, fmaf(m.y, pyf, t.x));  // S * texture coordinate
+          w2 = fmaf(m.z, pxf, fmaf(m.w, pyf, t.y));
+          return fmaxf(fabsf(u), fabsf(w2)) < h;
+        } else {
+          const float dx = fmaf(sx, pxf, t.x), dy = fmaf(sy, pyf, t.y);
+          u = fmaf(m.x, dx, m.y * dy);
+          w2 = fmaf(m.z, dx, m.w * dy);
+          return fmaxf(fabsf(dx), fabsf(dy)) < h;
+        }
       };
       auto shade = [&](float u, float w2, float opacity, const float4& col) {
         float alpha = opacity * spl_exp2(-fmaf(u, u, w2 * w2));
