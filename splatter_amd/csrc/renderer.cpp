@@ -262,6 +262,8 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   const int bits = r->depth_bits;
   const int radix = 1 << bits;
   const int Q = b.pair_passes;
+  int tile_bits = 1;  // bits of the largest tile key, max_tiles - 1
+  while (tile_bits < 31 && (b.max_tiles - 1u) >> tile_bits) ++tile_bits;
   const uint32_t cus = (uint32_t)r->dev->cu_count;
   const uint32_t per_view = std::max<uint32_t>(1, cus * 4 / nv);
   // radix-sort tiles: one 1024-lane, two 512-lane or four 256-lane blocks per CU, shared
@@ -330,7 +332,8 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.threads = r->sort_threads;
     a.grid_x = clampg(div_up(n, sort_tile), sort_per_view);
     a.views = (int)nv;
-    launch_sort_pass(bits, a, q);
+    // keys are below 2^30 (0 < clip.z < 1): the last 8-bit pass needs 6 bits
+    launch_sort_pass(std::min(bits, 30 - p * bits), a, q);
   }
   const int fin = P & 1;
   mark(3);
@@ -406,7 +409,8 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.threads = r->sort_threads;
     a.grid_x = clampg(div_up(r->pair_cap, sort_tile), sort_per_view);
     a.views = (int)nv;
-    launch_sort_pass(8, a, q);
+    // tile keys are below max_tiles: the last pass takes only their top bits
+    launch_sort_pass(std::max(5, std::min(8, tile_bits - 8 * p)), a, q);
   }
   const int pfin = Q & 1;
   mark(6);

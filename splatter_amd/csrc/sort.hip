@@ -258,23 +258,28 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4))) vo
   finish_persistent(tile_counter, counters + kDone + a.tile_slot);
 }
 
-void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s) {
+template <int BITS>
+void launch_sort_pass_as(const SortPassLaunch& a, hipStream_t s) {
   const dim3 grid(a.grid_x, a.views);
-  if (a.threads == kSortThreadsXL) {
-    if (bits == 10)
-      hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreadsXL, kSortItems>), grid, dim3(kSortThreadsXL), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreadsXL, kSortItems>), grid, dim3(kSortThreadsXL), 0, s, a);
-  } else if (a.threads == kSortThreadsWide) {
-    if (bits == 10)
-      hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreadsWide, kSortItems>), grid, dim3(kSortThreadsWide), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreadsWide, kSortItems>), grid, dim3(kSortThreadsWide), 0, s, a);
-  } else {
-    if (bits == 10)
-      hipLaunchKernelGGL((k_onesweep_pass<10, kSortThreads, kSortItems>), grid, dim3(kSortThreads), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_onesweep_pass<8, kSortThreads, kSortItems>), grid, dim3(kSortThreads), 0, s, a);
+  if (a.threads == kSortThreadsXL)
+    hipLaunchKernelGGL((k_onesweep_pass<BITS, kSortThreadsXL, kSortItems>), grid, dim3(kSortThreadsXL), 0, s, a);
+  else if (a.threads == kSortThreadsWide)
+    hipLaunchKernelGGL((k_onesweep_pass<BITS, kSortThreadsWide, kSortItems>), grid, dim3(kSortThreadsWide), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_onesweep_pass<BITS, kSortThreads, kSortItems>), grid, dim3(kSortThreads), 0, s, a);
+}
+
+// bits: the digit width of this pass, 5..8 or 10.  A pass over the top
+// bits of a key narrower than the other passes' digits takes a narrower
+// digit (fewer ballots, a shorter look-back); its histogram is the first
+// 2^bits bins of the full-width one, since the bits above are zero.
+void launch_sort_pass(int bits, const SortPassLaunch& a, hipStream_t s) {
+  switch (bits) {
+    case 5: launch_sort_pass_as<5>(a, s); break;
+    case 6: launch_sort_pass_as<6>(a, s); break;
+    case 7: launch_sort_pass_as<7>(a, s); break;
+    case 10: launch_sort_pass_as<10>(a, s); break;
+    default: launch_sort_pass_as<8>(a, s); break;
   }
 }
 
