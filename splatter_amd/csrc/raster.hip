@@ -74,6 +74,8 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   uint32_t* pair_keys = a.pair_keys + v * a.pair_stride;
   uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
   uint64_t* status = a.status + v * a.status_stride;
+  if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
+  if (a.clear_b) clear_words(a.clear_b + v * a.clear_b_stride, a.clear_b_words);
   for (int i = tid; i < a.pair_passes * 256; i += THREADS) s_hist[i] = 0u;
   const uint32_t n = counters[kVisible];
   const uint32_t num_tiles = (n + TILE - 1) / TILE;

@@ -122,6 +122,10 @@ struct spl_renderer {
   // or one of its own (a server slot)
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // the projection's branch of a sorted batch: it needs only the cull, so
+  // it runs beside the depth sort and the binning waits for both
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   uint64_t bytes = 0;  // device memory held
 };
 
@@ -175,6 +179,11 @@ void renderer_free(spl_renderer* r) {
   r->ev_pool.clear();
   r->ev_views.clear();
   r->ev_used = 0;
+  if (r->fork) (void)hipEventDestroy(r->fork);
+  if (r->join) (void)hipEventDestroy(r->join);
+  if (r->side) (void)hipStreamDestroy(r->side);
+  r->fork = r->join = nullptr;
+  r->side = nullptr;
   if (r->own_stream && r->stream) (void)hipStreamDestroy(r->stream);
   r->stream = nullptr;
   r->own_stream = false;
@@ -276,6 +285,10 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   const size_t sort_used = (size_t)div_up(n, kSortThreads * kSortItems) * radix;
   const size_t cull_used = div_up(n, kCullTile);
   auto mark = [&](int i) { if (ev) (void)hipEventRecord(ev[i], q); };
+  // A sorted batch projects on the side stream while the depth sort runs
+  // (the timing pass keeps one stream, so each stage's events bracket it
+  // alone).
+  const bool fork = sorted && !ev && r->side != nullptr;
   auto clampg = [](uint64_t want, uint32_t hi) { return (int)std::max<uint64_t>(1, std::min<uint64_t>(want, hi)); };
 
   mark(0);
@@ -296,7 +309,34 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.grid = clampg(div_up(n, kCullTile), cus * 4);
     launch_cull(a, q);
   }
+  auto launch_proj = [&](hipStream_t qq) {
+    ProjectLaunch a{};
+    a.scene = SceneView{s->centers, s->rotations, s->scale_alpha, s->sh, n};
+    a.views = nv;
+    a.uniforms = r->d_uniforms;
+    a.vis = r->vis;
+    a.tile_prefix = r->tile_prefix; a.tp_stride = r->cull_tiles;
+    a.sh_order = (int)cfg.spherical_harmonics_order;
+    a.use_covariance_for_scale = cfg.use_covariance_for_scale ? 1 : 0;
+    a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
+    a.records = r->records; a.rec_stride = cap * kRecordFloat4;
+    a.boxes = r->boxes; a.box_stride = cap;
+    a.conics = b.conics ? r->conics : nullptr;
+    if (P == 0) {  // (sorted, the histograms clear the cull's words and the binning the sort's)
+      a.clear_a = r->cull_status;
+      a.clear_a_words = cull_used;
+      a.clear_a_stride = r->cull_tiles;
+    }
+    a.grid = clampg(div_up(n, kCullTile), cus * 3);  // three resident blocks per CU
+    launch_project(a, qq);
+  };
   mark(1);
+  if (fork) {
+    (void)hipEventRecord(r->fork, q);
+    (void)hipStreamWaitEvent(r->side, r->fork, 0);
+    launch_proj(r->side);
+    (void)hipEventRecord(r->join, r->side);
+  }
   if (sorted) {
     HistLaunch a{};
     a.keys = r->keys[0]; a.keys_stride = cap;
@@ -337,37 +377,19 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   }
   const int fin = P & 1;
   mark(3);
+  if (!fork) launch_proj(q);
+  mark(4);
+  if (fork) (void)hipStreamWaitEvent(q, r->join, 0);
   {
-    ProjectLaunch a{};
-    a.scene = SceneView{s->centers, s->rotations, s->scale_alpha, s->sh, n};
-    a.views = nv;
-    a.uniforms = r->d_uniforms;
-    a.vis = r->vis;
-    a.tile_prefix = r->tile_prefix; a.tp_stride = r->cull_tiles;
-    a.sh_order = (int)cfg.spherical_harmonics_order;
-    a.use_covariance_for_scale = cfg.use_covariance_for_scale ? 1 : 0;
-    a.use_unaligned_rectangles = cfg.use_unaligned_rectangles ? 1 : 0;
-    a.records = r->records; a.rec_stride = cap * kRecordFloat4;
-    a.boxes = r->boxes; a.box_stride = cap;
-    a.conics = b.conics ? r->conics : nullptr;
-    if (P > 0) {
+    BinLaunch a{};
+    if (P > 0) {  // the depth sort's last status words and its histograms
       a.clear_a = r->sort_status + (size_t)(P - 1) * r->sort_words;
       a.clear_a_words = sort_used;
       a.clear_a_stride = sort_stride;
       a.clear_b = r->depth_hist;
       a.clear_b_words = (size_t)kMaxDepthPasses * 1024;
       a.clear_b_stride = (size_t)kMaxDepthPasses * 1024;
-    } else {
-      a.clear_a = r->cull_status;
-      a.clear_a_words = cull_used;
-      a.clear_a_stride = r->cull_tiles;
     }
-    a.grid = clampg(div_up(n, kCullTile), cus * 3);  // three resident blocks per CU
-    launch_project(a, q);
-  }
-  mark(4);
-  {
-    BinLaunch a{};
     a.order = sorted ? r->vals[fin] : nullptr; a.order_stride = cap;
     a.boxes = r->boxes; a.box_stride = cap;
     a.uniforms = r->d_uniforms;
@@ -676,6 +698,9 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
     delete r;
     return SPL_OK;
   }
+  chk(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+  chk(hipEventCreateWithFlags(&r->fork, hipEventDisableTiming));
+  chk(hipEventCreateWithFlags(&r->join, hipEventDisableTiming));
   for (auto& sl : r->slots) {
     chk(hipHostMalloc((void**)&sl.uniforms, V * sizeof(Uniforms), hipHostMallocDefault));
     chk(hipHostMalloc((void**)&sl.pixels, V * sizeof(uint32_t*), hipHostMallocDefault));

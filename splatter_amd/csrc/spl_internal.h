@@ -136,6 +136,9 @@ struct BinLaunch {
   int pair_passes;
   uint64_t* status; size_t status_stride;
   uint32_t* counters;
+  // finished bookkeeping of the depth sort, zeroed here (per view)
+  uint32_t* clear_a; size_t clear_a_words, clear_a_stride;
+  uint32_t* clear_b; size_t clear_b_words, clear_b_stride;
   int grid_x, views;
 };
 

@@ -64,6 +64,13 @@ def test_server_mix_equals_alone_and_plain():
                        memory_budget_bytes=2 << 30)
     scenes, records = make_scenes(server)
     reqs = requests(16, seed=5)
+    # A first pass of the same requests creates the slots, frames and launch
+    # graphs, so the measured burst below is submitted faster than the device
+    # renders it: requests then wait for a slot and leave in batches.
+    for t in [server.submit(scenes[s], w, h, cam, view) for s, w, h, (cam, view) in reqs]:
+        server.wait(t)
+        server.release(t)
+    warm = server.stats()
     tickets = [server.submit(scenes[s], w, h, cam, view) for s, w, h, (cam, view) in reqs]
     mixed = []
     for t in tickets:
@@ -75,8 +82,8 @@ def test_server_mix_equals_alone_and_plain():
         server.release(t)
     st = server.stats()
     say("stats after the mix:", st)
-    assert st["completed"] == 16 and st["failed"] == 0
-    assert st["batches"] < 16  # same-scene requests went out together
+    assert st["completed"] == 32 and st["failed"] == 0
+    assert st["batches"] - warm["batches"] < 16  # same-scene requests went out together
     assert st["memory_peak"] <= st["memory_budget"]
     alone = []
     for s, w, h, (cam, view) in reqs:
