@@ -3,12 +3,18 @@
 Workload: 6M clustered Gaussians (64 anisotropic clusters with 80% of the
 splats + a 20% ground slab), SH degree 3, 1920x1080, a 256-pose orbit
 (radius 6, elevation 20 deg), 16 views per batched launch.  One step = one
-batch of 16 views.  Output: one JSON line (see the keys below).
+batch of 16 views.  The batched loop keeps two batches in flight (two
+renderers with their own frames, alternating, as the serving form's slots
+do), so one batch's raster overlaps the next one's sort and projection; the
+plain path renders one view at a time on one renderer.  Output: one JSON
+line (see the keys below).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--in-flight 2]
 
 Per-stage device times come from HIP events that the library records on its
-own stream around every stage of every frame of the timed region.
+own stream around every stage of every frame of a second pass over the same
+batches on one renderer (one batch at a time, so each stage's events
+bracket it alone).
 """
 from __future__ import annotations
 
@@ -204,6 +210,9 @@ def main() -> int:
     ap.add_argument("--sort-tile", type=int, default=0, help="force keys per sort tile (skips the autotuner)")
     ap.add_argument("--no-autotune", action="store_true", help="8-bit digits, 4096-key sort tiles")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--in-flight", type=int, default=2,
+                    help="batches in flight: the batched loop alternates this many renderers, each with its own "
+                         "frames, so one batch's raster overlaps the next one's sort and projection")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="0 skips the serving measurement")
     ap.add_argument("--serving-sweep", type=str, default="",
@@ -257,20 +266,34 @@ def main() -> int:
         log(f"autotune picked {tuned.best.key()}")
         r = sa.Renderer(dev, cfg)
     frames = [sa.Frame(dev, W, H) for _ in range(V_PER)]
+    # batches in flight: renderer 0 is on the device stream (it also runs
+    # the plain path, the stage timing and the counters below), the others
+    # on streams of their own; each has its own frames
+    in_flight = 1 if args.profile else max(1, args.in_flight)
+    pipe = [r] + [sa.Renderer(dev, cfg, own_stream=True) for _ in range(in_flight - 1)]
+    pipe_frames = [frames] + [[sa.Frame(dev, W, H) for _ in range(V_PER)] for _ in range(in_flight - 1)]
 
-    def batch(k):
+    def sync_all():
+        dev.synchronize()
+        for rr in pipe[1:]:
+            rr.synchronize()
+
+    def batch(k, renderers=None):
+        renderers = renderers or pipe
+        j = k % len(renderers)
         sel = [poses[(k * V_PER + i) % len(poses)] for i in range(V_PER)]
-        r.render_views(frames, [p[0] for p in sel], [p[1] for p in sel], scene)
+        renderers[j].render_views(pipe_frames[j], [p[0] for p in sel], [p[1] for p in sel], scene)
 
-    for k in range(args.warmup):
+    for k in range(max(args.warmup, in_flight)):
         batch(k)
-    dev.synchronize()
+    sync_all()
 
-    # batched: K steps of V_PER views, one graph replay per step
+    # batched: K steps of V_PER views, one graph replay per step, in_flight
+    # steps on the device at a time
     t0 = time.perf_counter()
     for k in range(args.steps):
         batch(k)
-    dev.synchronize()
+    sync_all()
     t_batched = time.perf_counter() - t0
     batched_fps = args.steps * V_PER / t_batched
     log(f"batched: {batched_fps:.1f} frames/s")
@@ -299,11 +322,16 @@ def main() -> int:
     plain_fps = n_plain / (time.perf_counter() - t0)
     log(f"plain: {plain_fps:.1f} frames/s")
 
+    for rr in pipe[1:]:
+        rr.close()
+    pipe = [r]
+
     # per-stage device time per frame: HIP events around every stage of every
-    # batch of a second timed region of the same work (direct launches)
+    # batch of a second timed region of the same work (direct launches, one
+    # renderer, so each stage's events bracket it alone)
     r.enable_timing(True)
     for k in range(args.steps):
-        batch(k)
+        batch(k, [r])
     stage_ms = r.stage_times()
     r.enable_timing(False)
     r.enable_timing(True)
@@ -377,7 +405,7 @@ def main() -> int:
                    "splats": args.splats, "width": W, "height": H, "views_per_batch": V_PER,
                    "radix_bits_per_digit": cfg.radix_bits_per_digit, "sort_tile_keys": cfg.sort_tile_keys,
                    "raster_batch": cfg.raster_batch, "raster_longest_first": cfg.raster_longest_first,
-                   "raster_mode": cfg.raster_mode,
+                   "raster_mode": cfg.raster_mode, "batches_in_flight": in_flight,
                    "autotune": tuned.to_json() if tuned else None},
         "plain_frames_per_s": round(plain_fps, 2),
         "batched_over_plain": round(batched_fps / plain_fps, 3),

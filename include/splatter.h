@@ -116,6 +116,17 @@ int spl_synchronize(spl_device* device);
  * workspace for config->max_splat_count splats. */
 int spl_renderer_create(spl_device* device, const spl_configuration* config, spl_renderer** out);
 int spl_renderer_destroy(spl_renderer* renderer);
+
+/* No counterpart in the reference; for callers that keep several batches in
+ * flight.  A renderer whose launches go to a stream of its own instead of
+ * the device's, so that the batches of two such renderers run on the device
+ * at the same time (the serving form's slots are such renderers).  Scenes,
+ * frames and their uploads are complete when their calls return, so it needs
+ * no ordering with the device stream.  spl_synchronize does not wait for
+ * it: spl_renderer_synchronize does, and must come before reading a frame it
+ * rendered. */
+int spl_renderer_create_async(spl_device* device, const spl_configuration* config, spl_renderer** out);
+int spl_renderer_synchronize(spl_renderer* renderer);
 int spl_renderer_get_config(const spl_renderer* renderer, spl_configuration* out);
 
 /* scene.rs:45 Scene::new: splat_count is clamped to max_splat_count (scene.rs:46). */

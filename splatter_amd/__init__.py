@@ -171,13 +171,22 @@ class Frame:
 class Renderer:
     """renderer.rs:64-84 Renderer."""
 
-    def __init__(self, device: Device, config: Configuration | None = None):
+    def __init__(self, device: Device, config: Configuration | None = None, own_stream: bool = False):
+        """own_stream: launches go to a stream of the renderer's own
+        (spl_renderer_create_async), so the batches of two such renderers
+        overlap on the device; call synchronize() before reading its frames."""
         self.config = config or Configuration()
         self.device = device
         h = C.c_void_p()
         cc = self.config.to_c()
-        check(lib().spl_renderer_create(device.handle, C.byref(cc), C.byref(h)))
+        create = lib().spl_renderer_create_async if own_stream else lib().spl_renderer_create
+        check(create(device.handle, C.byref(cc), C.byref(h)))
         self._h = h
+        self.own_stream = own_stream
+
+    def synchronize(self) -> None:
+        """Waits for everything this renderer has queued."""
+        check(lib().spl_renderer_synchronize(self._h))
 
     @property
     def handle(self):

@@ -96,6 +96,8 @@ SIGNATURES = [
     ("spl_synchronize", C.c_int, [_P]),
     ("spl_renderer_create", C.c_int, [_P, C.POINTER(SplConfiguration), _PP]),
     ("spl_renderer_destroy", C.c_int, [_P]),
+    ("spl_renderer_create_async", C.c_int, [_P, C.POINTER(SplConfiguration), _PP]),
+    ("spl_renderer_synchronize", C.c_int, [_P]),
     ("spl_renderer_get_config", C.c_int, [_P, C.POINTER(SplConfiguration)]),
     ("spl_scene_create", C.c_int, [_P, C.c_uint64, _PP]),
     ("spl_scene_new_dev_test", C.c_int, [_P, _PP]),

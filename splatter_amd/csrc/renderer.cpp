@@ -755,6 +755,17 @@ int spl_renderer_create(spl_device* dev, const spl_configuration* cfg, spl_rende
   return renderer_create(dev, cfg, false, 0, out, nullptr);
 }
 
+int spl_renderer_create_async(spl_device* dev, const spl_configuration* cfg, spl_renderer** out) {
+  return renderer_create(dev, cfg, true, 0, out, nullptr);
+}
+
+int spl_renderer_synchronize(spl_renderer* r) {
+  if (!r) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_synchronize: null renderer");
+  HIP_TRY(hipSetDevice(r->dev->ordinal));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return SPL_OK;
+}
+
 int spl_renderer_destroy(spl_renderer* r) {
   if (!r) return SPL_OK;
   (void)hipSetDevice(r->dev->ordinal);
