@@ -15,7 +15,7 @@ import pytest
 import oracle_bind as ob
 import splatter_amd as sa
 from gpu_common import Case, config, device, image_diff, normwise_rel, say
-from splatter_amd.synth import Workload, look_at, orbit_poses, sphere_pose
+from splatter_amd.synth import Workload, load_scene, look_at, orbit_poses, sphere_pose
 
 pytestmark = pytest.mark.gpu
 
@@ -199,3 +199,35 @@ def test_render_views_equals_single_frames(graphs, longest):
     u = ob.uniforms(cam, view, *sizes[2])
     _, _, _, oimg = ob.render_frame(case.splats, u, 3, True)
     assert image_diff(batched[2][..., [2, 1, 0, 3]], oimg) <= IMAGE_TOL
+
+
+def test_batches_in_flight_on_async_renderers():
+    """Two renderers on streams of their own, alternating batches with their
+    own frames (two batches on the device at once), give exactly the images
+    of one renderer on the device stream."""
+    w = Workload("clustered", 200_000, seed=10)
+    cfg = config(max_views=4)
+    poses = orbit_poses(12)
+    sizes = [(320, 240), (640, 360), (97, 61), (320, 240)]
+    ref_r = sa.Renderer(device(), cfg)
+    ref_s = sa.Scene(ref_r, w.count)
+    load_scene(ref_s, w)
+    ref = []
+    for k in range(3):
+        fr = [sa.Frame(device(), wd, ht) for wd, ht in sizes]
+        sel = poses[4 * k:4 * k + 4]
+        ref_r.render_views(fr, [p[0] for p in sel], [p[1] for p in sel], ref_s)
+        ref.append([f.read() for f in fr])
+    rs = [sa.Renderer(device(), cfg, own_stream=True) for _ in range(2)]
+    scene = sa.Scene(rs[0], w.count)
+    load_scene(scene, w)
+    fsets = [[sa.Frame(device(), wd, ht) for wd, ht in sizes] for _ in range(3)]
+    for rep in range(2):  # second round replays the captured graphs
+        for k in range(3):
+            sel = poses[4 * k:4 * k + 4]
+            rs[k % 2].render_views(fsets[k], [p[0] for p in sel], [p[1] for p in sel], scene)
+        for r in rs:
+            r.synchronize()
+        for k in range(3):
+            for got, want in zip((f.read() for f in fsets[k]), ref[k]):
+                assert np.array_equal(got, want)
