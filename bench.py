@@ -261,7 +261,8 @@ def main() -> int:
         # each candidate timed on the same resident scene
         from splatter_amd import autotune as at
         r.close()
-        tuned = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=3, log=log)
+        tuned = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=4, log=log,
+                                   in_flight=max(1, args.in_flight))
         tuned.apply(cfg)
         log(f"autotune picked {tuned.best.key()}")
         r = sa.Renderer(dev, cfg)

@@ -74,8 +74,11 @@ def candidates(radix_bits=RADIX_BITS, sort_tile_keys=SORT_TILE_KEYS, batch_views
 
 
 def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand: Candidate,
-            steps: int = 3) -> float:
-    """frames/s of `steps` graph-replayed batches of cand.max_views views."""
+            steps: int = 3, in_flight: int = 1) -> float:
+    """frames/s of `steps` graph-replayed batches of cand.max_views views,
+    with `in_flight` renderers (the first on the device stream, the others
+    on streams of their own, each with its own frames) taking alternate
+    batches."""
     cfg = type(base)(**{k: getattr(base, k) for k in base.__dataclass_fields__})
     cfg.radix_bits_per_digit = cand.radix_bits_per_digit
     cfg.sort_tile_keys = cand.sort_tile_keys
@@ -83,25 +86,33 @@ def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand
     cfg.raster_mode = cand.raster_mode
     cfg.raster_batch = cand.raster_batch
     cfg.raster_longest_first = cand.raster_longest_first
-    r = sa.Renderer(device, cfg)
+    rs = [sa.Renderer(device, cfg)] + [sa.Renderer(device, cfg, own_stream=True) for _ in range(in_flight - 1)]
     try:
-        scene = scene_loader(r)
-        frames = [sa.Frame(device, width, height) for _ in range(cand.max_views)]
+        scene = scene_loader(rs[0])
         nv = cand.max_views
+        fsets = [[sa.Frame(device, width, height) for _ in range(nv)] for _ in rs]
 
         def batch(k):
+            j = k % len(rs)
             sel = [poses[(k * nv + i) % len(poses)] for i in range(nv)]
-            r.render_views(frames, [p[0] for p in sel], [p[1] for p in sel], scene)
+            rs[j].render_views(fsets[j], [p[0] for p in sel], [p[1] for p in sel], scene)
 
-        batch(0)
-        device.synchronize()
+        def sync():
+            device.synchronize()
+            for r in rs[1:]:
+                r.synchronize()
+
+        for k in range(len(rs)):
+            batch(k)
+        sync()
         t0 = time.perf_counter()
         for k in range(steps):
-            batch(k + 1)
-        device.synchronize()
+            batch(k + len(rs))
+        sync()
         return steps * nv / (time.perf_counter() - t0)
     finally:
-        r.close()
+        for r in rs:
+            r.close()
 
 
 def autotune(sa, device, base, scene_loader, poses, width: int, height: int, cands=None, steps: int = 3,
@@ -144,16 +155,16 @@ def store_cached(path: str, key: str, result: TuneResult) -> None:
 
 
 def autotune_staged(sa, device, base, scene_loader, poses, width: int, height: int, views: int, steps: int = 3,
-                    log=None, walk=(0,)) -> TuneResult:
+                    log=None, walk=(0,), in_flight: int = 1) -> TuneResult:
     """Coordinate search: the sort's digit width x tile with the raster at
     its defaults, then the raster's batch x tile order (x walk) with the best
-    sort, at a fixed batch width."""
+    sort, at a fixed batch width and number of batches in flight."""
     table = []
 
     def run(cands):
         best = None
         for c in cands:
-            fps = measure(sa, device, base, scene_loader, poses, width, height, c, steps)
+            fps = measure(sa, device, base, scene_loader, poses, width, height, c, steps, in_flight)
             table.append((c.key(), round(fps, 2)))
             if log:
                 log(f"autotune {c.key()}: {fps:.1f} frames/s")
