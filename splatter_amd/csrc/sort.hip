@@ -10,7 +10,10 @@
 //    reference ranks with LDS atomicAdd (shaders.wgsl:428-429), which is not.
 //  - tiles are TILE = THREADS*ITEMS keys (4096, 8192 or 16384) taken in order from an
 //    atomic counter by a persistent grid (reference: 1024-key tiles, one
-//    workgroup each, shaders.wgsl:406-420).
+//    workgroup each, shaders.wgsl:406-420); full tiles skip every per-key
+//    bounds test.
+//  - the digit width is per pass (5..8 or 10 bits): a last pass over the
+//    top bits of a narrower key takes only those bits.
 //  - the global digit offsets come from the upsweep histogram and are
 //    scanned by every block on entry (radixSortB's serial scan disappears).
 //  - the views of a batch sort side by side (view = blockIdx.y), each with
