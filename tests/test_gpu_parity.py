@@ -231,3 +231,30 @@ def test_batches_in_flight_on_async_renderers():
         for k in range(3):
             for got, want in zip((f.read() for f in fsets[k]), ref[k]):
                 assert np.array_equal(got, want)
+
+
+def test_batch_wider_than_sixteen_views():
+    """max_views = 32: the cull's 32-bit visibility masks and the
+    projection's second half of views (views 16..31); every view of a
+    20-view batch equals its single render, entries and image."""
+    w = Workload("gaussian", 60_000, seed=12)
+    poses = orbit_poses(20, radius=4)
+    r = sa.Renderer(device(), config(max_views=32, max_splat_count=w.count))
+    scene = sa.Scene(r, w.count)
+    load_scene(scene, w)
+    frames = [sa.Frame(device(), 160 + 8 * (i % 3), 120) for i in range(20)]
+    r.render_views(frames, [p[0] for p in poses], [p[1] for p in poses], scene)
+    batched = [f.read() for f in frames]
+    entries = []
+    for vi in (0, 15, 16, 19):
+        r.select_view(vi)
+        entries.append((vi, *r.read_entries()))
+    r.select_view(0)
+    for vi, keys, idx in entries:
+        cam, view = poses[vi]
+        u = ob.uniforms(cam, view, frames[vi].width, frames[vi].height)
+        okeys, oidx = ob.cpu_depth_sort(ob.decode(w.all())["center"], ob.vp_of(u), 1.1)
+        assert np.array_equal(idx, oidx) and np.array_equal(keys, okeys)
+    for (cam, view), img, f in zip(poses, batched, frames):
+        r.render_frame(f, cam, view, scene)
+        assert np.array_equal(f.read(), img)
