@@ -1,18 +1,27 @@
 // api.cpp -- the C ABI of include/splatter.h for devices, scenes and frames:
 // HIP device and stream, the scene's device layout and the record decode of
 // scene.rs:225-249 (load_chunk), the PLY header parse of scene.rs:197-213,
-// and render targets.  The renderer lives in renderer.cpp.
+// and render targets.  The renderer lives in renderer.cpp.  Scene uploads
+// read and decode records on host threads into pinned staging and copy 2^18
+// records at a time, one chunk's copy overlapping the next chunk's decode.
 #include "splatter.h"
 
 #include <hip/hip_runtime.h>
 
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <new>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "spl_internal.h"
@@ -190,22 +199,23 @@ static_assert(sizeof(SerializedSplat) == 248, "SerializedSplat is 62 f32");
 
 constexpr float kMaxSizeVariance = 5.0f;  // scene.rs:13
 
-// Host staging of decoded splats in the device layout.
-struct Staging {
-  std::vector<float> centers, sh;
-  std::vector<float4> rot, sa;
-  void resize(size_t n) {
-    centers.resize(n * 3);
-    sh.resize(n * 48);
-    rot.resize(n);
-    sa.resize(n);
-  }
+// Pinned host staging of decoded splats in the device layout, one slot per
+// chunk in flight.  `done` is recorded on the device stream after the slot's
+// copies, so the slot is refilled only once its previous chunk has landed.
+struct StagingSlot {
+  void* block = nullptr;
+  float* sh = nullptr;       // 48 n
+  float4* rot = nullptr;     // n
+  float4* sa = nullptr;      // n
+  float* centers = nullptr;  // 3 n
+  hipEvent_t done = nullptr;
+  bool pending = false;
 };
 
-// scene.rs:225-249.  Rotor::signum (geometric_algebra 0.3.0) is r / |r|.
-void decode(const SerializedSplat* in, size_t n, Staging& st) {
-  st.resize(n);
-  for (size_t i = 0; i < n; ++i) {
+// scene.rs:225-249 for records [lo, hi).  Rotor::signum (geometric_algebra
+// 0.3.0) is r / |r|.
+void decode_range(const SerializedSplat* in, size_t lo, size_t hi, StagingSlot& st) {
+  for (size_t i = lo; i < hi; ++i) {
     const SerializedSplat& s = in[i];
     const float* q = s.rotation;
     const float sq = ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3];
@@ -221,17 +231,152 @@ void decode(const SerializedSplat* in, size_t n, Staging& st) {
   }
 }
 
-int upload(spl_scene* s, uint64_t first, const Staging& st, size_t n) {
-  if (n == 0) return SPL_OK;
-  HIP_TRY(hipSetDevice(s->dev->ordinal));
-  hipStream_t q = s->dev->stream;
-  HIP_TRY(hipMemcpyAsync(s->centers + 3 * first, st.centers.data(), n * 12, hipMemcpyHostToDevice, q));
-  HIP_TRY(hipMemcpyAsync(s->rotations + first, st.rot.data(), n * 16, hipMemcpyHostToDevice, q));
-  HIP_TRY(hipMemcpyAsync(s->scale_alpha + first, st.sa.data(), n * 16, hipMemcpyHostToDevice, q));
-  HIP_TRY(hipMemcpyAsync(s->sh + 48 * first, st.sh.data(), n * 192, hipMemcpyHostToDevice, q));
-  HIP_TRY(hipStreamSynchronize(q));
-  return SPL_OK;
+// Runs fn(lo, hi) over [0, n) split into contiguous ranges on up to 16 host
+// threads (the calling thread takes the first range).
+void parallel_ranges(size_t n, const std::function<void(size_t, size_t)>& fn) {
+  constexpr size_t kPerThread = 1u << 14;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t workers = std::min<size_t>({(size_t)std::min(hw, 16u), (n + kPerThread - 1) / kPerThread, 16});
+  if (workers <= 1) {
+    fn((size_t)0, n);
+    return;
+  }
+  const size_t step = (n + workers - 1) / workers;
+  std::vector<std::thread> pool;
+  pool.reserve(workers - 1);
+  for (size_t w = 1; w < workers; ++w) {
+    const size_t lo = std::min(n, w * step), hi = std::min(n, lo + step);
+    pool.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+  }
+  fn((size_t)0, std::min(n, step));
+  for (auto& t : pool) t.join();
 }
+
+// Each record is decoded by one thread with the same arithmetic, so the result
+// does not depend on the split.
+void decode(const SerializedSplat* in, size_t n, StagingSlot& st) {
+  parallel_ranges(n, [&](size_t lo, size_t hi) { decode_range(in, lo, hi, st); });
+}
+
+// Reads `bytes` at `offset` of fd; false on an error or end of file first.
+bool read_at(int fd, void* dst, size_t bytes, uint64_t offset) {
+  char* p = static_cast<char*>(dst);
+  while (bytes) {
+    const ssize_t got = pread(fd, p, bytes, (off_t)offset);
+    if (got < 0 && errno == EINTR) continue;
+    if (got <= 0) return false;
+    p += got;
+    bytes -= (size_t)got;
+    offset += (uint64_t)got;
+  }
+  return true;
+}
+
+// Streams decoded chunks to a scene's device arrays.  Two pinned slots
+// alternate: the copies of chunk k run on the device stream while chunk k+1 is
+// decoded on the host.  finish() waits for the last copy; the destructor also
+// waits before the slots are freed.
+class Uploader {
+ public:
+  static constexpr size_t kChunk = 1u << 18;
+  static constexpr size_t kBytesPerSplat = 48 * 4 + 16 + 16 + 12;
+
+  Uploader(spl_scene* s, size_t max_chunk) : s_(s), cap_(std::max<size_t>(1, std::min(kChunk, max_chunk))) {}
+  Uploader(const Uploader&) = delete;
+  Uploader& operator=(const Uploader&) = delete;
+
+  ~Uploader() {
+    (void)hipSetDevice(s_->dev->ordinal);
+    for (auto& sl : slot_) {
+      if (sl.done) {
+        if (sl.pending) (void)hipEventSynchronize(sl.done);
+        (void)hipEventDestroy(sl.done);
+      }
+      if (sl.block) (void)hipHostFree(sl.block);
+    }
+  }
+
+  size_t capacity() const { return cap_; }
+
+  // The next slot, once the copies that last read it have finished.
+  int acquire(StagingSlot** out) {
+    HIP_TRY(hipSetDevice(s_->dev->ordinal));
+    StagingSlot& sl = slot_[next_];
+    if (!sl.block) {
+      hipError_t e = hipHostMalloc(&sl.block, cap_ * kBytesPerSplat, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        sl.block = nullptr;
+        return fail(SPL_ERROR_OUT_OF_MEMORY, std::string("pinned staging: ") + hipGetErrorString(e));
+      }
+      char* b = static_cast<char*>(sl.block);
+      sl.sh = reinterpret_cast<float*>(b);
+      sl.rot = reinterpret_cast<float4*>(b + cap_ * 192);
+      sl.sa = reinterpret_cast<float4*>(b + cap_ * 208);
+      sl.centers = reinterpret_cast<float*>(b + cap_ * 224);
+      HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+    if (sl.pending) {
+      HIP_TRY(hipEventSynchronize(sl.done));
+      sl.pending = false;
+    }
+    *out = &sl;
+    return SPL_OK;
+  }
+
+  // Queues the copies of the acquired slot's first n splats to [first, first+n).
+  int send(StagingSlot& sl, uint64_t first, size_t n) {
+    next_ ^= 1;
+    if (n == 0) return SPL_OK;
+    hipStream_t q = s_->dev->stream;
+    HIP_TRY(hipMemcpyAsync(s_->centers + 3 * first, sl.centers, n * 12, hipMemcpyHostToDevice, q));
+    HIP_TRY(hipMemcpyAsync(s_->rotations + first, sl.rot, n * 16, hipMemcpyHostToDevice, q));
+    HIP_TRY(hipMemcpyAsync(s_->scale_alpha + first, sl.sa, n * 16, hipMemcpyHostToDevice, q));
+    HIP_TRY(hipMemcpyAsync(s_->sh + 48 * first, sl.sh, n * 192, hipMemcpyHostToDevice, q));
+    HIP_TRY(hipEventRecord(sl.done, q));
+    sl.pending = true;
+    return SPL_OK;
+  }
+
+  // Decodes n <= capacity() records and queues their upload to [first, first+n).
+  int push(uint64_t first, const SerializedSplat* in, size_t n) {
+    StagingSlot* sl = nullptr;
+    int rc = acquire(&sl);
+    if (rc) return rc;
+    decode(in, n, *sl);
+    return send(*sl, first, n);
+  }
+
+  // As push, for n records read from fd at `offset` into buf: each thread
+  // reads and decodes its own range.
+  int push_file(uint64_t first, int fd, uint64_t offset, SerializedSplat* buf, size_t n) {
+    StagingSlot* sl = nullptr;
+    int rc = acquire(&sl);
+    if (rc) return rc;
+    std::atomic<bool> ok{true};
+    parallel_ranges(n, [&](size_t lo, size_t hi) {
+      if (!read_at(fd, buf + lo, (hi - lo) * sizeof(SerializedSplat), offset + lo * sizeof(SerializedSplat))) {
+        ok = false;
+        return;
+      }
+      decode_range(buf, lo, hi, *sl);
+    });
+    if (!ok) return fail(SPL_ERROR_IO, "short read in load_chunk");
+    return send(*sl, first, n);
+  }
+
+  int finish() {
+    HIP_TRY(hipSetDevice(s_->dev->ordinal));
+    HIP_TRY(hipStreamSynchronize(s_->dev->stream));
+    for (auto& sl : slot_) sl.pending = false;
+    return SPL_OK;
+  }
+
+ private:
+  spl_scene* s_;
+  size_t cap_;
+  StagingSlot slot_[2];
+  int next_ = 0;
+};
 
 }  // namespace
 
@@ -239,16 +384,15 @@ int spl_scene_load_records(spl_scene* s, uint64_t first, uint64_t count, const v
   if (!s || (!records && count)) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_scene_load_records: null argument");
   if (first > s->count || count > s->count - first)
     return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_scene_load_records: range beyond splat_count");
+  if (count == 0) return SPL_OK;
   const auto* in = static_cast<const SerializedSplat*>(records);
-  const size_t chunk = 1u << 18;
-  Staging st;
-  for (uint64_t done = 0; done < count; done += chunk) {
-    const size_t n = (size_t)std::min<uint64_t>(chunk, count - done);
-    decode(in + done, n, st);
-    int rc = upload(s, first + done, st, n);
+  Uploader up(s, (size_t)std::min<uint64_t>(count, Uploader::kChunk));
+  for (uint64_t done = 0; done < count; done += up.capacity()) {
+    const size_t n = (size_t)std::min<uint64_t>(up.capacity(), count - done);
+    int rc = up.push(first + done, in + done, n);
     if (rc) return rc;
   }
-  return SPL_OK;
+  return up.finish();
 }
 
 int spl_scene_new_dev_test(spl_renderer* r, spl_scene** out) {
@@ -256,20 +400,24 @@ int spl_scene_new_dev_test(spl_renderer* r, spl_scene** out) {
   if (rc) return rc;
   spl_scene* s = *out;
   // scene.rs:159-177: raw values, not decoded
-  Staging st;
-  st.resize(3);
-  for (int i = 0; i < 3; ++i) {
-    st.rot[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-    st.centers[3 * i + 0] = i == 0 ? 1.0f : 0.0f;
-    st.centers[3 * i + 1] = i == 1 ? 1.0f : 0.0f;
-    st.centers[3 * i + 2] = i == 2 ? 1.0f : 0.0f;
-    st.sa[i] = make_float4(0.2f, 0.1f, 0.05f, 1.0f);
-    for (int k = 0; k < 48; ++k) st.sh[48 * i + k] = 0.0f;
-    st.sh[48 * i + 0] = i == 0 ? 1.0f : 0.0f;
-    st.sh[48 * i + 1] = i == 1 ? 1.0f : 0.0f;
-    st.sh[48 * i + 2] = i == 2 ? 1.0f : 0.0f;
+  Uploader up(s, 3);
+  StagingSlot* st = nullptr;
+  rc = up.acquire(&st);
+  if (!rc) {
+    for (int i = 0; i < 3; ++i) {
+      st->rot[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+      st->centers[3 * i + 0] = i == 0 ? 1.0f : 0.0f;
+      st->centers[3 * i + 1] = i == 1 ? 1.0f : 0.0f;
+      st->centers[3 * i + 2] = i == 2 ? 1.0f : 0.0f;
+      st->sa[i] = make_float4(0.2f, 0.1f, 0.05f, 1.0f);
+      for (int k = 0; k < 48; ++k) st->sh[48 * i + k] = 0.0f;
+      st->sh[48 * i + 0] = i == 0 ? 1.0f : 0.0f;
+      st->sh[48 * i + 1] = i == 1 ? 1.0f : 0.0f;
+      st->sh[48 * i + 2] = i == 2 ? 1.0f : 0.0f;
+    }
+    rc = up.send(*st, 0, (size_t)std::min<uint64_t>(3, s->count));
   }
-  rc = upload(s, 0, st, (size_t)std::min<uint64_t>(3, s->count));
+  if (!rc) rc = up.finish();
   if (rc) {
     spl_scene_destroy(s);
     *out = nullptr;
@@ -315,22 +463,20 @@ int spl_scene_load_chunk(spl_scene* s, const char* path, uint16_t header_size, u
   if (!s || !path) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_scene_load_chunk: null argument");
   end = std::min<uint64_t>(end, s->count);  // scene.rs:217
   if (begin >= end) return SPL_OK;
-  std::ifstream f(path, std::ios::binary);
-  if (!f) return fail(SPL_ERROR_IO, std::string("cannot open ") + path);
-  f.seekg((std::streamoff)header_size + (std::streamoff)(begin * sizeof(SerializedSplat)));
-  const size_t chunk = 1u << 18;
-  std::vector<SerializedSplat> buf;
-  Staging st;
-  for (uint64_t at = begin; at < end; at += chunk) {
-    const size_t n = (size_t)std::min<uint64_t>(chunk, end - at);
-    buf.resize(n);
-    f.read(reinterpret_cast<char*>(buf.data()), (std::streamsize)(n * sizeof(SerializedSplat)));
-    if (!f) return fail(SPL_ERROR_IO, "short read in load_chunk");
-    decode(buf.data(), n, st);
-    int rc = upload(s, at, st, n);
-    if (rc) return rc;
+  const int fd = open(path, O_RDONLY | O_CLOEXEC);
+  if (fd < 0) return fail(SPL_ERROR_IO, std::string("cannot open ") + path);
+  int rc = SPL_OK;
+  {
+    Uploader up(s, (size_t)std::min<uint64_t>(end - begin, Uploader::kChunk));
+    std::vector<SerializedSplat> buf(up.capacity());
+    for (uint64_t at = begin; at < end && !rc; at += up.capacity()) {
+      const size_t n = (size_t)std::min<uint64_t>(up.capacity(), end - at);
+      rc = up.push_file(at, fd, (uint64_t)header_size + at * sizeof(SerializedSplat), buf.data(), n);
+    }
+    if (!rc) rc = up.finish();
   }
-  return SPL_OK;
+  close(fd);
+  return rc;
 }
 
 // ---------------------------------------------------------------------------
