@@ -155,6 +155,49 @@ def test_edge_cases_empty_culled_tiny_ragged():
         check_case(case, conics=False)
 
 
+def test_malformed_records():
+    """Records a PLY file may hold but no trainer writes: zero, infinite and
+    NaN rotations, centres, scales, opacity logits and colours, splats at the
+    camera and splats larger than the view.  The decode (scene.rs:225-249)
+    turns them into NaN or infinite attributes; the visible set and order
+    stay bit-exact, non-finite rectangles get no tiles, and the image stays
+    within the bar."""
+    w = Workload("gaussian", 20_000, seed=21)
+    cam, view = sphere_pose(21, radius=4)
+    case = Case(Workload("gaussian", 0, seed=0), config(), 320, 240, (cam, view))
+    rec = w.all().copy()
+    rec["rotation"][0:100] = 0.0
+    rec["center"][100:200] = np.nan
+    rec["scale"][200:300] = -np.inf
+    rec["scale"][300:400] = 1e3
+    rec["alpha"][400:450] = 1e30
+    rec["alpha"][450:500] = -1e30
+    rec["color"][500:600] = np.nan
+    rec["color"][600:650] = 1e30
+    rec["scale"][650:660] = 3.0
+    rec["alpha"][650:660] = -3.0
+    rec["center"][660:700] = cam[3, :3]
+    rec["scale"][700:800, 0] = -30.0
+    rec["rotation"][800:900] = np.inf
+    case.records, case.splats = rec, ob.decode(rec)
+    case.scene = sa.Scene(case.renderer, len(rec))
+    case.scene.load_records(0, rec)
+    img = case.render()
+    keys, idx = case.renderer.read_entries()
+    okeys, oidx, orec, oimg = case.oracle()
+    say(f"  visible gpu={len(idx)} oracle={len(oidx)}")
+    assert np.array_equal(idx, oidx) and np.array_equal(keys, okeys)
+    got = case.renderer.read_records()
+    assert np.array_equal(got[:, :4], orec[:, :4], equal_nan=True)
+    fin = np.all(np.isfinite(orec[:, 4:]), axis=1)
+    assert (~fin).sum() > 300 and np.isnan(orec[:, :4]).any()
+    assert np.array_equal(fin, np.all(np.isfinite(got[:, 4:]), axis=1))
+    assert normwise_rel(got[fin, 4:], orec[fin, 4:]).max() <= RECORD_TOL
+    d = image_diff(img, oimg)
+    say(f"  image max diff {d}/255")
+    assert d <= IMAGE_TOL and (img[..., 3] < 255).mean() > 0.2
+
+
 def test_deterministic_and_frame_size_changes():
     w = Workload("clustered", 400_000, seed=6)
     case = Case(w, config(), 1920, 1080, orbit_poses(16)[5])
