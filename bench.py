@@ -32,6 +32,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "6M-splat 1080p frames/s batched vs plain; sort Gkeys/s; p50/p99 render latency"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per SIMD every 4 cycles
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 STAGES = ["cull_key_compact", "digit_histograms", "depth_sort", "project", "bin", "pair_sort",
           "tile_ranges", "raster"]
 
@@ -63,6 +65,27 @@ STAGE_KERNELS = {"cull_key_compact": "k_cull_views", "digit_histograms": "k_digi
                  "depth_sort": "k_onesweep_pass", "project": "k_project", "bin": "k_bin",
                  "pair_sort": "k_onesweep_pass", "tile_ranges": "k_tile_ranges", "raster": "k_raster"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_6M_clustered_1080p_16views.txt")
+KERNELS_FILE = os.path.join(ROOT, "profiles", "kernels_6M_clustered_1080p_16views.md")
+
+
+def measured_valu(stage: str):
+    """(kernel, SQ_INSTS_VALU per launch) of the stage's kernel from the
+    committed rocprofv3 counter summary of this workload, or None."""
+    if not os.path.exists(KERNELS_FILE):
+        return None
+    want, cols, in_sq = STAGE_KERNELS.get(stage), None, False
+    for line in open(KERNELS_FILE):
+        if line.startswith("## "):
+            in_sq = line.startswith("## SQ counters")
+            continue
+        f = line.split()
+        if not in_sq or not f:
+            continue
+        if f[0] == "kernel":
+            cols = f
+        elif cols and "INSTS_VALU" in cols and f[0].startswith(want):
+            return f[0], float(f[cols.index("INSTS_VALU")])
+    return None
 
 
 def measured_traffic(stage: str):
@@ -371,6 +394,16 @@ def main() -> int:
     dom_name = STAGES[dom]
     launches = {"depth_sort": P, "pair_sort": Q}.get(dom_name, 1)
     achieved = nbytes[dom_name] / (stage_ms[dom] * 1e-3) / 1e9
+    # the raster is bound by VALU issue rather than HBM: its wave instructions
+    # per launch (counters of the same workload) over the live launch time
+    valu = measured_valu(dom_name)
+    valu_issue = None
+    if valu and launches == 1:
+        rate = valu[1] / (stage_ms[dom] * 1e-3 * V_PER)
+        valu_issue = {"kernel": valu[0], "instructions_per_launch": valu[1], "achieved": round(rate / 1e9, 1),
+                      "peak": round(VALU_PEAK_PER_S / 1e9, 1), "unit": "G wave64 VALU instructions/s",
+                      "frac": round(rate / VALU_PEAK_PER_S, 4),
+                      "source": "SQ_INSTS_VALU per launch in profiles/kernels_6M_clustered_1080p_16views.md"}
     sort_i = STAGES.index("depth_sort")
     sort_gkeys = V / (stage_ms[sort_i] * 1e-3) / 1e9 if stage_ms[sort_i] > 0 else None
 
@@ -424,7 +457,8 @@ def main() -> int:
         "raster_tiles": tiles_info,
         "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_batch": launches,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name),
+                     "valu_issue": valu_issue},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
