@@ -21,6 +21,7 @@
 #include <new>
 #include <functional>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -244,10 +245,16 @@ void parallel_ranges(size_t n, const std::function<void(size_t, size_t)>& fn) {
   const size_t step = (n + workers - 1) / workers;
   std::vector<std::thread> pool;
   pool.reserve(workers - 1);
-  for (size_t w = 1; w < workers; ++w) {
-    const size_t lo = std::min(n, w * step), hi = std::min(n, lo + step);
-    pool.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+  size_t w = 1;
+  try {
+    for (; w < workers; ++w) {
+      const size_t lo = std::min(n, w * step), hi = std::min(n, lo + step);
+      pool.emplace_back([&fn, lo, hi] { fn(lo, hi); });
+    }
+  } catch (const std::system_error&) {
+    // no thread to be had: the calling thread takes the remaining ranges
   }
+  for (size_t r = w; r < workers; ++r) fn(std::min(n, r * step), std::min(n, r * step + step));
   fn((size_t)0, std::min(n, step));
   for (auto& t : pool) t.join();
 }
