@@ -22,6 +22,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -240,6 +241,8 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="0 skips the serving measurement")
     ap.add_argument("--serving-sweep", type=str, default="",
                     help="comma-separated request rates: run only the serving measurement at each and print them")
+    ap.add_argument("--tune-only", action="store_true",
+                    help="run the autotuner on the workload and print its result as one JSON line")
     ap.add_argument("--profile", action="store_true",
                     help="batched steps and the stage-timing pass only (every launch has the batch's shape), "
                          "then three 256 MiB device-to-device copies whose counters scale FETCH_SIZE/WRITE_SIZE")
@@ -251,7 +254,28 @@ def main() -> int:
         return 2
 
     import splatter_amd as sa
+    from splatter_amd import autotune as at
     from splatter_amd.synth import Workload, load_scene, orbit_poses
+
+    W, H, V_PER = args.width, args.height, args.views
+    tuned = None
+    if not (args.no_autotune or args.radix_bits or args.sort_tile or args.profile or args.tune_only
+            or args.serving_sweep):
+        # The tuner creates and frees two renderers per candidate, and with
+        # them their streams.  Batches in flight overlapped fully only on
+        # renderers whose streams came first in the process: 1540 frames/s
+        # against about 1390 after the tuner had run in the same process
+        # (MI355X, two batches in flight).  So the tuner runs in a child
+        # process, started before this one makes any HIP call.
+        cmd = [sys.executable, os.path.abspath(__file__), "--tune-only", "--splats", str(args.splats),
+               "--views", str(V_PER), "--width", str(W), "--height", str(H), "--in-flight", str(args.in_flight)]
+        child = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=sys.stderr, text=True, timeout=900)
+        lines = [x for x in child.stdout.splitlines() if x.startswith("{")]
+        if child.returncode != 0 or not lines:
+            print(f"autotune child failed with status {child.returncode}", file=sys.stderr)
+            return 1
+        tuned = at.TuneResult.from_json(json.loads(lines[-1]))
+        log(f"autotune picked {tuned.best.key()}")
 
     if args.serving_sweep:
         dev = sa.Device(0)
@@ -263,44 +287,50 @@ def main() -> int:
             print(json.dumps(res), flush=True)
         return 0
 
-    W, H, V_PER = args.width, args.height, args.views
     L = 3
     dev = sa.Device(0)
     cfg = sa.Configuration(spherical_harmonics_order=L, max_splat_count=args.splats, max_views=V_PER,
                            radix_bits_per_digit=args.radix_bits or 8, sort_tile_keys=args.sort_tile or 4096)
-    r = sa.Renderer(dev, cfg)
+    if tuned:
+        tuned.apply(cfg)
+    in_flight = 1 if (args.profile or args.tune_only) else max(1, args.in_flight)
+    if in_flight == 1:
+        # renderer 0, on the device stream, takes the batched steps too
+        r = sa.Renderer(dev, cfg)
+        pipe = [r]
+    else:
+        # each batch in flight on a renderer with a stream of its own, made
+        # before any other renderer (batches on the device stream did not
+        # overlap with those of an own-stream renderer: 1282 against 1535
+        # frames/s for two in flight); renderer 0 comes after the batched
+        # steps
+        r = None
+        pipe = [sa.Renderer(dev, cfg, own_stream=True) for _ in range(in_flight)]
     wl = Workload("clustered", args.splats, seed=2)
     t0 = time.perf_counter()
-    scene = sa.Scene(r, args.splats)
+    scene = sa.Scene(pipe[0], args.splats)
     positions = np.empty((args.splats, 3), np.float32)
     for first, rec in wl.chunks():
         scene.load_records(first, rec)
         positions[first:first + len(rec)] = rec["center"]
     log(f"scene: {args.splats} splats generated+uploaded in {time.perf_counter() - t0:.1f}s")
     poses = orbit_poses(256)
-    tuned = None
-    if not (args.no_autotune or args.radix_bits or args.sort_tile or args.profile):
+    if args.tune_only:
         # radix digit width x sort tile (block shape) at this batch width,
         # each candidate timed on the same resident scene
-        from splatter_amd import autotune as at
-        r.close()
-        tuned = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=4, log=log,
-                                   in_flight=max(1, args.in_flight))
-        tuned.apply(cfg)
-        log(f"autotune picked {tuned.best.key()}")
-        r = sa.Renderer(dev, cfg)
+        pipe[0].close()
+        res = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=4, log=log,
+                                 in_flight=max(1, args.in_flight))
+        print(json.dumps(res.to_json()), flush=True)
+        return 0
     frames = [sa.Frame(dev, W, H) for _ in range(V_PER)]
-    # batches in flight: renderer 0 is on the device stream (it also runs
-    # the plain path, the stage timing and the counters below), the others
-    # on streams of their own; each has its own frames
-    in_flight = 1 if args.profile else max(1, args.in_flight)
-    pipe = [r] + [sa.Renderer(dev, cfg, own_stream=True) for _ in range(in_flight - 1)]
-    pipe_frames = [frames] + [[sa.Frame(dev, W, H) for _ in range(V_PER)] for _ in range(in_flight - 1)]
+    pipe_frames = [frames] + [[sa.Frame(dev, W, H) for _ in range(V_PER)] for _ in range(len(pipe) - 1)]
 
     def sync_all():
         dev.synchronize()
-        for rr in pipe[1:]:
-            rr.synchronize()
+        for rr in pipe:
+            if rr.own_stream:
+                rr.synchronize()
 
     def batch(k, renderers=None):
         renderers = renderers or pipe
@@ -322,17 +352,26 @@ def main() -> int:
     batched_fps = args.steps * V_PER / t_batched
     log(f"batched: {batched_fps:.1f} frames/s")
 
+    # per-stage device time per frame: HIP events around every stage of every
+    # batch of a second timed region of the same work on the first batched
+    # renderer alone (direct launches, so each stage's events bracket it alone)
+    t_r = pipe[0]
+    t_r.enable_timing(True)
+    for k in range(args.steps):
+        batch(k, [t_r])
+    stage_ms = t_r.stage_times()
+    t_r.enable_timing(False)
+
     if args.profile:
-        r.enable_timing(True)
-        for k in range(args.steps):
-            batch(k)
-        stage_ms = r.stage_times()
-        r.enable_timing(False)
         copy_calibration()
         print(json.dumps({"profile": True, "batched_frames_per_s": round(batched_fps, 2), "views_per_batch": V_PER,
                           "stage_ms_per_frame": {s: round(float(stage_ms[i]), 4) for i, s in enumerate(STAGES)}}))
         return 0
 
+    # renderer 0 on the device stream: the plain path, the stage timing and
+    # the counters
+    if r is None:
+        r = sa.Renderer(dev, cfg)
     # plain: the same views, one render_frame each
     n_plain = args.steps * V_PER
     for i in range(2):
@@ -346,18 +385,14 @@ def main() -> int:
     plain_fps = n_plain / (time.perf_counter() - t0)
     log(f"plain: {plain_fps:.1f} frames/s")
 
-    for rr in pipe[1:]:
-        rr.close()
-    pipe = [r]
+    for k, rr in enumerate(pipe):
+        if rr is not r:
+            rr.close()
+            if k > 0:
+                for f in pipe_frames[k]:
+                    f.close()
+    pipe, pipe_frames = [r], [frames]
 
-    # per-stage device time per frame: HIP events around every stage of every
-    # batch of a second timed region of the same work (direct launches, one
-    # renderer, so each stage's events bracket it alone)
-    r.enable_timing(True)
-    for k in range(args.steps):
-        batch(k, [r])
-    stage_ms = r.stage_times()
-    r.enable_timing(False)
     r.enable_timing(True)
     for i in range(min(n_plain, 64)):
         r.render_frame(frames[0], *poses[i % len(poses)], scene)

@@ -76,9 +76,10 @@ def candidates(radix_bits=RADIX_BITS, sort_tile_keys=SORT_TILE_KEYS, batch_views
 def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand: Candidate,
             steps: int = 3, in_flight: int = 1) -> float:
     """frames/s of `steps` graph-replayed batches of cand.max_views views,
-    with `in_flight` renderers (the first on the device stream, the others
-    on streams of their own, each with its own frames) taking alternate
-    batches."""
+    with `in_flight` renderers taking alternate batches, each with its own
+    frames.  One renderer runs on the device stream.  With more than one,
+    each runs on a stream of its own: batches on the device stream did not
+    overlap with those of an own-stream renderer, measured on MI355X."""
     cfg = type(base)(**{k: getattr(base, k) for k in base.__dataclass_fields__})
     cfg.radix_bits_per_digit = cand.radix_bits_per_digit
     cfg.sort_tile_keys = cand.sort_tile_keys
@@ -86,7 +87,8 @@ def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand
     cfg.raster_mode = cand.raster_mode
     cfg.raster_batch = cand.raster_batch
     cfg.raster_longest_first = cand.raster_longest_first
-    rs = [sa.Renderer(device, cfg)] + [sa.Renderer(device, cfg, own_stream=True) for _ in range(in_flight - 1)]
+    rs = ([sa.Renderer(device, cfg)] if in_flight <= 1 else
+          [sa.Renderer(device, cfg, own_stream=True) for _ in range(in_flight)])
     try:
         scene = scene_loader(rs[0])
         nv = cand.max_views
@@ -99,8 +101,9 @@ def measure(sa, device, base, scene_loader, poses, width: int, height: int, cand
 
         def sync():
             device.synchronize()
-            for r in rs[1:]:
-                r.synchronize()
+            for r in rs:
+                if r.own_stream:
+                    r.synchronize()
 
         for k in range(len(rs)):
             batch(k)
