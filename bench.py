@@ -319,7 +319,7 @@ def main() -> int:
         # radix digit width x sort tile (block shape) at this batch width,
         # each candidate timed on the same resident scene
         pipe[0].close()
-        res = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=4, log=log,
+        res = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=8, log=log,
                                  in_flight=max(1, args.in_flight))
         print(json.dumps(res.to_json()), flush=True)
         return 0

@@ -82,6 +82,15 @@ int spl_device_open(int ordinal, spl_device** out) {
     delete d;
     return fail(SPL_ERROR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
+  for (int i = 0; i < kPoolStreams; ++i) {
+    e = hipStreamCreateWithFlags(&d->pool[i], hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      for (int j = 0; j < i; ++j) (void)hipStreamDestroy(d->pool[j]);
+      (void)hipStreamDestroy(d->stream);
+      delete d;
+      return fail(SPL_ERROR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+  }
   (void)hipDeviceGetAttribute(&d->cu_count, hipDeviceAttributeMultiprocessorCount, ordinal);
   if (d->cu_count <= 0) d->cu_count = 256;
   *out = d;
@@ -121,7 +130,29 @@ void spl::device_release(spl_device* d) {
   (void)hipSetDevice(d->ordinal);
   (void)hipStreamSynchronize(d->stream);
   (void)hipStreamDestroy(d->stream);
+  for (hipStream_t s : d->pool) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
   delete d;
+}
+
+hipStream_t spl::acquire_stream(spl_device* d) {
+  std::lock_guard<std::mutex> lock(d->pool_mu);
+  int best = 0;
+  for (int i = 1; i < kPoolStreams; ++i)
+    if (d->pool_users[i] < d->pool_users[best]) best = i;
+  ++d->pool_users[best];
+  return d->pool[best];
+}
+
+void spl::release_stream(spl_device* d, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(d->pool_mu);
+  for (int i = 0; i < kPoolStreams; ++i)
+    if (d->pool[i] == s && d->pool_users[i] > 0) {
+      --d->pool_users[i];
+      return;
+    }
 }
 
 int spl::scene_create(spl_device* dev, uint64_t count, spl_scene** out) {

@@ -181,10 +181,16 @@ void renderer_free(spl_renderer* r) {
   r->ev_used = 0;
   if (r->fork) (void)hipEventDestroy(r->fork);
   if (r->join) (void)hipEventDestroy(r->join);
-  if (r->side) (void)hipStreamDestroy(r->side);
+  if (r->side) {
+    (void)hipStreamSynchronize(r->side);
+    release_stream(r->dev, r->side);
+  }
   r->fork = r->join = nullptr;
   r->side = nullptr;
-  if (r->own_stream && r->stream) (void)hipStreamDestroy(r->stream);
+  if (r->own_stream && r->stream) {
+    (void)hipStreamSynchronize(r->stream);
+    release_stream(r->dev, r->stream);
+  }
   r->stream = nullptr;
   r->own_stream = false;
 }
@@ -641,11 +647,7 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   r->cfg = *cfg;
   r->stream = dev->stream;
   if (own_stream && !dry) {
-    hipError_t se = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
-    if (se != hipSuccess) {
-      delete r;
-      return fail(SPL_ERROR_DEVICE, std::string("spl_renderer_create: stream: ") + hipGetErrorString(se));
-    }
+    r->stream = acquire_stream(dev);
     r->own_stream = true;
   }
   if (r->cfg.tile_pairs_per_splat == 0) r->cfg.tile_pairs_per_splat = 4;
@@ -698,7 +700,7 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
     delete r;
     return SPL_OK;
   }
-  chk(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+  r->side = acquire_stream(dev);
   chk(hipEventCreateWithFlags(&r->fork, hipEventDisableTiming));
   chk(hipEventCreateWithFlags(&r->join, hipEventDisableTiming));
   for (auto& sl : r->slots) {
