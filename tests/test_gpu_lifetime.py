@@ -53,3 +53,35 @@ def test_server_outlives_its_device_handle():
     server.release(req)
     server.close()
     scene.close()
+
+
+def test_pair_capacity_overflow_is_reported_and_recoverable():
+    """More (splat, tile) pairs than tile_pairs_per_splat * max_splat_count:
+    the bin keeps the pairs that fit, the frame completes, pair_count
+    reports SPL_ERROR_CAPACITY, and the next frame that fits is exact."""
+    dev = sa.Device(0)
+    cfg = config(max_splat_count=20_000, tile_pairs_per_splat=1)
+    r = sa.Renderer(dev, cfg)
+    big = Workload("gaussian", 20_000, seed=31).all().copy()
+    big["scale"] += 2.5  # log-scales: every splat spans many 16-pixel tiles
+    s_big = sa.Scene(r, len(big))
+    s_big.load_records(0, big)
+    f = sa.Frame(dev, 1280, 720)
+    cam, view = sphere_pose(31, radius=4)
+    say("overflowing frame")
+    r.render_frame(f, cam, view, s_big)
+    with pytest.raises(sa.SplatterError):
+        r.pair_count()
+    assert f.read().shape == (720, 1280, 4)
+
+    small = Workload("gaussian", 2_000, seed=32)
+    s_small = sa.Scene(r, small.count)
+    load_scene(s_small, small)
+    r.render_frame(f, cam, view, s_small)
+    assert 0 < r.pair_count() <= 20_000
+    after = f.read()
+    fresh = sa.Renderer(dev, cfg)
+    fresh.render_frame(f, cam, view, s_small)
+    assert np.array_equal(after, f.read())
+    for o in (fresh, s_small, s_big, f, r):
+        o.close()
