@@ -4,8 +4,8 @@
  * Drop-in for the per-frame path of Lichtso/splatter: every entry point below
  * names the Rust item it replaces.  A Rust crate keeps the reference's public
  * API (`splatter::renderer::{Renderer, Configuration, DepthSorting}`,
- * `splatter::scene::Scene`) and forwards to these functions; the binding is
- * spelled out in INTEGRATION.md.
+ * `splatter::scene::Scene`) and forwards to these functions through an
+ * `extern "C"` block that declares them as written here.
  *
  * Conventions
  *  - Opaque handles; every call returns an int status (SPL_OK == 0, negative
