@@ -97,8 +97,10 @@ typedef struct spl_configuration {
                                              1: each lane walks the entries covering its pixel;
                                              2: four lane groups step through the lists of their 4x4
                                              sub-blocks side by side (all the same image) */
-    uint32_t raster_batch;                /* entries a raster block stages per batch: 128 or 256 */
-    uint32_t raster_longest_first;        /* 1: tiles with the longest lists start first (same image) */
+    uint32_t raster_batch;                /* entries a raster block stages per batch: 128 or 256
+                                             (batches of two or more views; one view uses 256) */
+    uint32_t raster_longest_first;        /* 1: tiles with the longest lists start first (same image;
+                                             batches of two or more views; one view always does) */
 } spl_configuration;
 
 const char* spl_last_error(void);
