@@ -505,7 +505,11 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           // branch is needed.
           alpha = alpha >= 1.0f / 255.0f ? alpha : 0.0f;
           const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
-          const float sa = clamp01(alpha);
+          // the source alpha needs no clamp: the opacity is a sigmoid (or 1
+          // in the dev-test scene) and spl_exp2 of a non-positive argument is
+          // at most 1 (p(f) <= p(0) = 1 when rint(y) = 0, p(f) / 2 < 1 below),
+          // so alpha is in [0, 1] after the threshold
+          const float sa = alpha;
           c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
           c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
           c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
