@@ -72,8 +72,8 @@ class Configuration:
     use_hip_graphs: bool = True
     sort_tile_keys: int = 4096
     raster_mode: int = 0
-    raster_batch: int = 128
-    raster_longest_first: int = 0
+    raster_batch: int = 128          # batches of two or more views; one view rasters with 256
+    raster_longest_first: int = 0    # batches of two or more views; one view always starts longest tiles first
 
     def to_c(self) -> SplConfiguration:
         c = SplConfiguration()
