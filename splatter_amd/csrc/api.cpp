@@ -161,7 +161,12 @@ int spl::scene_create(spl_device* dev, uint64_t count, spl_scene** out) {
   HIP_TRY(hipSetDevice(dev->ordinal));
   auto* s = new (std::nothrow) spl_scene();
   if (!s) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_scene_create: host allocation");
+  // A destroyed scene's host address (and device buffers) can come back for
+  // the next scene; the id never does, so a renderer's cached launch
+  // sequences, which bake the scene's device pointers in, are keyed on it.
+  static std::atomic<uint64_t> next_uid{1};
   s->dev = dev;
+  s->uid = next_uid.fetch_add(1, std::memory_order_relaxed);
   s->count = count;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };

@@ -42,6 +42,9 @@ __global__ __launch_bounds__(THREADS) void k_cull_views(CullLaunch a) {
   const uint32_t n = a.n;
   const uint32_t num_tiles = (n + TILE - 1) / TILE;
   if (num_tiles == 0 && blockIdx.x == 0 && tid < nv) a.counters[(size_t)tid * kCounterWords + kVisible] = 0u;
+  // The consistency flag that the sort's scatter and the binning raise
+  // describes one batch: the cull, first in the sequence, lowers it.
+  if (blockIdx.x == 0 && tid < nv) a.counters[(size_t)tid * kCounterWords + kScatterError] = 0u;
 
   // visibility of one centre in view v, and its depth key
   auto classify = [&](uint32_t v, float x, float y, float z, uint32_t& key) -> bool {

@@ -535,7 +535,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           float4 m[U], t[U], col[U];
 #pragma unroll
           for (int k = 0; k < U; ++k) {
-            const uint32_t j = (js >> (8 * k)) & 255u;  // slots past the list hold stale, in-range indices
+            // slots past the list may be stale or, on a block's first batch,
+            // uninitialised: masked into [0, B) they still index s_ent, and
+            // their fragments are never blended (i + k < mine)
+            const uint32_t j = (js >> (8 * k)) & (uint32_t)(B - 1);
             m[k] = s_ent[buf][0][j];
             t[k] = s_ent[buf][1][j];
             col[k] = s_ent[buf][2][j];
@@ -556,7 +559,10 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
           float4 m[U], t[U], col[U];
 #pragma unroll
           for (int k = 0; k < U; ++k) {
-            const uint32_t j = (js >> (8 * k)) & 255u;  // slots past n hold stale, in-range indices
+            // slots past n may be stale or, on a block's first batch,
+            // uninitialised: masked into [0, B) they still index s_ent, and
+            // the loop below stops at n before using them
+            const uint32_t j = (js >> (8 * k)) & (uint32_t)(B - 1);
             m[k] = s_ent[buf][0][j];
             t[k] = s_ent[buf][1][j];
             col[k] = s_ent[buf][2][j];

@@ -33,6 +33,7 @@ struct UploadSlot {
 // Everything a captured launch sequence bakes into its kernel arguments.
 struct BatchShape {
   const spl_scene* scene = nullptr;
+  uint64_t scene_uid = 0;  // the scene's process-unique id: its address may be reused after it is destroyed
   uint64_t n = 0;
   uint32_t views = 0;
   uint32_t max_tiles = 0;
@@ -40,7 +41,7 @@ struct BatchShape {
   bool conics = false;
   bool tile_stats = false;
   bool operator==(const BatchShape& o) const {
-    return scene == o.scene && n == o.n && views == o.views && max_tiles == o.max_tiles &&
+    return scene_uid == o.scene_uid && n == o.n && views == o.views && max_tiles == o.max_tiles &&
            pair_passes == o.pair_passes && conics == o.conics && tile_stats == o.tile_stats;
   }
 };
@@ -537,6 +538,7 @@ int enqueue_batch(spl_renderer* r, spl_scene* s, uint32_t nv, spl_frame* const* 
 
   BatchShape b;
   b.scene = s;
+  b.scene_uid = s->uid;
   b.n = s->count;
   b.views = nv;
   b.max_tiles = max_tiles;

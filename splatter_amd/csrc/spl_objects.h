@@ -88,6 +88,7 @@ void release_stream(spl_device* d, hipStream_t s);
 
 struct spl_scene {
   spl_device* dev = nullptr;
+  uint64_t uid = 0;  // unique for the life of the process: names the scene in cached launch sequences
   uint64_t count = 0;
   float* centers = nullptr;
   float4* rotations = nullptr;

@@ -85,3 +85,37 @@ def test_pair_capacity_overflow_is_reported_and_recoverable():
     assert np.array_equal(after, f.read())
     for o in (fresh, s_small, s_big, f, r):
         o.close()
+
+
+def test_destroyed_scene_does_not_replay_its_launch_sequence():
+    """A renderer caches one captured launch sequence per batch shape, with the
+    scene's device buffers baked in.  Scene A is rendered, destroyed, and
+    scene B of the same splat count (different contents) is created on the
+    same renderer, often at A's old host address: B's frame must equal a
+    fresh renderer's frame of B, several times over."""
+    dev = sa.Device(0)
+    cfg = config(max_splat_count=30_000)
+    r = sa.Renderer(dev, cfg)
+    f = sa.Frame(dev, 320, 240)
+    cam, view = sphere_pose(41, radius=4)
+    wa, wb = Workload("gaussian", 30_000, seed=41), Workload("clustered", 30_000, seed=42)
+    for rnd in range(3):
+        say(f"destroy-then-recreate round {rnd}")
+        a = sa.Scene(r, wa.count)
+        load_scene(a, wa)
+        r.render_frame(f, cam, view, a)
+        img_a = f.read()
+        a.close()
+        b = sa.Scene(r, wb.count)
+        load_scene(b, wb)
+        r.render_frame(f, cam, view, b)
+        img_b = f.read()
+        fresh = sa.Renderer(dev, cfg)
+        fresh.render_frame(f, cam, view, b)
+        want = f.read()
+        fresh.close()
+        b.close()
+        assert np.array_equal(img_b, want)
+        assert not np.array_equal(img_a, want)
+    r.close()
+    f.close()

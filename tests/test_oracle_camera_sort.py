@@ -5,6 +5,9 @@ dev-test scene's known answer (ties broken by index), brute force in float64
 away from the culling boundary, invariants of the sorted entries, numpy's
 stable argsort as an independent stable sort, and a planted mistake
 (division in place of reciprocal-multiply) that the key must not follow.
+The vertex-stage predicate of DepthSorting::None (shaders.wgsl:54-61) is
+pinned on points at the frustum's edges against an f32 restatement, with the
+planted division by h.w alone shown to give a different visible set.
 """
 import math
 
@@ -151,3 +154,43 @@ def test_vertex_stage_predicate_differs_from_fallback_only_at_the_boundary():
     assert np.all(np.diff(a.astype(np.int64)) > 0)
     sym = np.setxor1d(a, np.sort(b))
     assert len(sym) <= 3
+
+
+def test_vertex_stage_predicate_divides_by_w_plus_1e7():
+    """DepthSorting::None culls with h.xyz / (h.w + 1e-7) (shaders.wgsl:54-61).
+    Points placed on the x and y edges of the frustum at near depths, where
+    the 1e-7 moves the quotient by a fraction of an ulp: the oracle's visible
+    set is exactly the f32 restatement with the 1e-7, and the planted
+    division by h.w alone gives a different set."""
+    cam, view = sphere_pose(6, radius=3.0)
+    u = ob.uniforms(cam, view, 1280, 720)
+    vp = ob.vp_of(u)
+    V = vp.reshape(4, 4).astype(np.float64)
+    rng = np.random.default_rng(23)
+    n = 400_000
+    tol = 1.1
+    # view-space points on the clip-space edges |c.x| = tol or |c.y| = tol
+    cam64 = cam.astype(np.float64)
+    C3, e = cam64[:3, :3].T, cam64[3, :3]
+    z = rng.uniform(1.05, 1.6, n)
+    edge = rng.choice([-1.0, 1.0], n) * tol * (1.0 + rng.normal(0, 3e-7, n))
+    other = rng.uniform(-0.9, 0.9, n) * tol
+    on_x = rng.uniform(size=n) < 0.5
+    cx = np.where(on_x, edge, other)
+    cy = np.where(on_x, other, edge)
+    vx, vy = cx * z * u.view_size[0], cy * z * u.view_size[1]
+    pos = (np.stack([vx, vy, z], 1) @ C3.T + e).astype(np.float32)
+    got = ob.gpu_predicate_cull(pos, vp, tol)
+    Vf = V.astype(np.float32)
+    h = [((Vf[0, r] * pos[:, 0] + Vf[1, r] * pos[:, 1]) + Vf[2, r] * pos[:, 2]) + Vf[3, r] * np.float32(1)
+         for r in range(4)]
+
+    def visible(w):
+        c = [(h[r] / w).astype(np.float32) for r in range(3)]
+        return np.nonzero((np.abs(c[0]) < np.float32(tol)) & (np.abs(c[1]) < np.float32(tol))
+                          & (np.abs(c[2] - np.float32(0.5)) < np.float32(0.5)))[0].astype(np.uint32)
+
+    want = visible((h[3] + np.float32(1e-7)).astype(np.float32))
+    planted = visible(h[3])
+    assert np.array_equal(got, want)
+    assert len(np.setxor1d(got, planted)) >= 10, len(np.setxor1d(got, planted))

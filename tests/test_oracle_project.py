@@ -7,7 +7,11 @@
 - quaternion convention: a 90 degree turn about z swaps the x/y scales;
 - the record's rectangle (translation, axis directions, semi-axes) equals the
   conic's centre and eigen-decomposition from numpy (textbook routines);
-- FP32 restatement agrees with FP64 on the covariance path.
+- FP32 restatement agrees with FP64 on the covariance path;
+- off the optical axis (x/z, y/z beyond the +-1 clamp too), the covariance
+  path's semi-axes equal an independent numpy EWA (eigenvalues of
+  J C^T R S^2 R^T C J^T), with planted mistakes (sign of J[0][2], no clamp)
+  shown to miss.
 """
 import math
 
@@ -159,3 +163,74 @@ def test_fp32_matches_fp64_on_covariance_path():
     assert np.array_equal(r64[:, :4], r32[:, :4])  # colour is f32 in both
     scale = np.abs(r64[:, 4:]).max(axis=1, keepdims=True)
     assert np.abs(r64[:, 4:] - r32[:, 4:]).max() < 2e-3 * scale.max()
+
+
+def ewa_semi_axes(center, scale, rot, cam, clamp=True, j02_sign=-1.0):
+    """Independent float64 EWA restatement of the covariance scale path
+    (shaders.wgsl:99-118, 240-246): world covariance R S^2 R^T, the view
+    position from the camera's own axes and eye (not the view matrix), the
+    perspective Jacobian at the position with x/z and y/z clamped to
+    [-1, 1], and the square roots of the eigenvalues of the projected 2x2
+    covariance from numpy's symmetric eigensolver.  clamp and j02_sign exist
+    to plant mistakes."""
+    r, i, j, k = np.asarray(rot, np.float64)
+    R = np.array([[1 - 2 * (j * j + k * k), 2 * (i * j - r * k), 2 * (i * k + r * j)],
+                  [2 * (i * j + r * k), 1 - 2 * (i * i + k * k), 2 * (j * k - r * i)],
+                  [2 * (i * k - r * j), 2 * (j * k + r * i), 1 - 2 * (i * i + j * j)]])
+    S2 = np.diag(np.asarray(scale, np.float64) ** 2)
+    sigma = R @ S2 @ R.T
+    C3 = cam[:3, :3].astype(np.float64).T  # columns: camera axes in world space
+    e = cam[3, :3].astype(np.float64)
+    x, y, z = C3.T @ (np.asarray(center, np.float64) - e)
+    if clamp:
+        x = np.clip(x / z, -1.0, 1.0) * z
+        y = np.clip(y / z, -1.0, 1.0) * z
+    J = np.array([[1 / z, 0.0, j02_sign * x / (z * z)], [0.0, 1 / z, -y / (z * z)]])
+    cov2 = J @ (C3.T @ sigma @ C3) @ J.T
+    lam = np.linalg.eigvalsh(cov2)
+    return np.sqrt(np.maximum(lam[::-1], 0.0))  # major, minor
+
+
+def off_axis_splats(rng, cam, n):
+    """Splats in front of the camera at view-space x/z in [-1.8, 1.8] and y/z
+    in [-1.5, 1.5] (a third of them beyond the clamp), anisotropic, turned."""
+    C3 = cam[:3, :3].astype(np.float64).T
+    e = cam[3, :3].astype(np.float64)
+    out = []
+    for _ in range(n):
+        z = rng.uniform(2.0, 10.0)
+        v = np.array([rng.uniform(-1.8, 1.8) * z, rng.uniform(-1.5, 1.5) * z, z])
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        out.append(splat((C3 @ v + e).astype(np.float32), np.exp(rng.normal(-3.0, 0.6, 3)).astype(np.float32),
+                         q.astype(np.float32)))
+    return out
+
+
+def test_covariance_semi_axes_off_axis_match_independent_ewa():
+    """The covariance scale path (the showcase default) pinned off the optical
+    axis, where the Jacobian's third column and the +-1 clamp matter: the
+    record's semi-axes |col| - bias equal the independent EWA restatement's.
+    Planted mistakes in the restatement -- a flipped sign of J[0][2] and a
+    dropped clamp -- each disagree with the oracle, so the same mistakes in
+    the oracle would fail this test."""
+    rng = np.random.default_rng(17)
+    worst, flipped, unclamped, beyond = 0.0, 0.0, 0.0, 0
+    for t in range(6):
+        cam, view = sphere_pose(300 + t, radius=6.0)
+        u = ob.uniforms(cam, view, 1920, 1080)
+        for sp in off_axis_splats(rng, cam, 40):
+            rec, _ = ob.project(sp, u, 0, covariance=True)
+            r = rec[0].astype(np.float64)
+            got = np.array([np.hypot(r[4], r[5]), np.hypot(r[6], r[7])]) - u.ellipse_size_bias
+            args = (sp["center"][0], sp["scale"][0], sp["rotation"][0], cam)
+            want = ewa_semi_axes(*args)
+            worst = max(worst, float(np.abs(got - want).max() / want[0]))
+            flipped = max(flipped, float(np.abs(ewa_semi_axes(*args, j02_sign=1.0) - got).max() / want[0]))
+            unclamped = max(unclamped, float(np.abs(ewa_semi_axes(*args, clamp=False) - got).max() / want[0]))
+            C3 = cam[:3, :3].astype(np.float64).T
+            pv = C3.T @ (sp["center"][0].astype(np.float64) - cam[3, :3])
+            beyond += int(abs(pv[0] / pv[2]) > 1.0 or abs(pv[1] / pv[2]) > 1.0)
+    assert beyond > 50
+    assert worst < 1e-5, worst
+    assert flipped > 1e-2 and unclamped > 1e-2, (flipped, unclamped)

@@ -7,7 +7,9 @@ renderer.rs:260-283,442).
   C = c1 a1 + (1 - a1) c2 a2, T = (1 - a1)(1 - a2);
 - the fragment threshold alpha < 1/255 drops a splat;
 - front-to-back: an opaque first splat hides the second;
-- the pixel-box walk equals a brute-force walk over every pixel.
+- the pixel-box walk equals a brute-force walk over every pixel;
+- the axis-aligned square (use_unaligned_rectangles = false) equals its
+  analytic footprint, with the planted min-for-max radius shown to miss.
 """
 import math
 
@@ -109,3 +111,44 @@ def test_bbox_walk_equals_brute_force():
 def test_clear_is_black_with_unit_alpha():
     img = ob.raster(np.zeros((0, 10), np.float32), uni())
     assert (img[..., :3] == 0).all() and (img[..., 3] == 255).all()
+
+
+def square_footprint(u, rec, margin=2.0, radius_of=max):
+    """Float64 analytic footprint of an axis-aligned square splat
+    (shaders.wgsl:562-571): pixel centres q with |q - t|_inf < radius *
+    margin, radius = max(|col0|, |col1|), texture coordinate A^-1 (q - t),
+    alpha = a exp(-|tc|^2 / 2).  radius_of plants the min-for-max mistake."""
+    a, col0, col1, t = rec[3], rec[4:6].astype(np.float64), rec[6:8].astype(np.float64), rec[8:10]
+    qx, qy = pixel_q(u)
+    dx, dy = qx - t[0], qy - t[1]
+    radius = radius_of(np.linalg.norm(col0), np.linalg.norm(col1))
+    inside = (np.abs(dx) < radius * margin) & (np.abs(dy) < radius * margin)
+    A = np.stack([col0, col1], axis=1)
+    Ai = np.linalg.inv(A)
+    tcx = Ai[0, 0] * dx + Ai[0, 1] * dy
+    tcy = Ai[1, 0] * dx + Ai[1, 1] * dy
+    alpha = a * np.exp(-0.5 * (tcx ** 2 + tcy ** 2))
+    return inside & (alpha >= 1 / 255), alpha
+
+
+def test_axis_aligned_square_analytic_footprint():
+    """use_unaligned_rectangles = false: the square quad around an elongated,
+    turned splat.  The oracle's image equals the analytic footprint within one
+    code; the planted min-for-max radius would crop the splat's long axis and
+    miss the oracle by far more."""
+    u = uni()
+    rec = record((0.8, 0.4, 0.2), 0.9, 0.3, 0.08, (0.1, -0.05), 0.6)
+    img = ob.raster(rec[None], u, unaligned=False, blend_f32=True)
+    drawn, alpha = square_footprint(u, rec)
+    want_r = np.where(drawn, np.round(np.clip(0.8 * alpha, 0, 1) * 255), 0)
+    want_t = np.where(drawn, np.round((1 - alpha) * 255), 255)
+    assert drawn.sum() > 300
+    assert np.abs(img[..., 0].astype(int) - want_r).max() <= 1
+    assert np.abs(img[..., 3].astype(int) - want_t).max() <= 1
+    # the square covers more than the rotated rectangle would
+    assert (img[..., 3] < 255).sum() > (ob.raster(rec[None], u, unaligned=True, blend_f32=True)[..., 3] < 255).sum()
+    planted, _ = square_footprint(u, rec, radius_of=min)
+    want_planted = np.where(planted, np.round(np.clip(0.8 * alpha, 0, 1) * 255), 0)
+    assert np.abs(img[..., 0].astype(int) - want_planted).max() > 20
+    img8 = ob.raster(rec[None], u, unaligned=False, blend_f32=False)
+    assert np.abs(img8.astype(int) - img.astype(int)).max() <= 1
