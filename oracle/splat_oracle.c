@@ -8,6 +8,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -495,6 +496,207 @@ double orc_time_cpu_depth_sort(const float* positions, size_t count, const float
     for (int i = 0; i < repeats; ++i) {
         double t0 = now_s();
         v = orc_cpu_depth_sort(positions, count, vp, tol, keys, idx);
+        t[i] = now_s() - t0;
+    }
+    qsort(t, (size_t)repeats, sizeof(double), cmp_double);
+    double med = repeats > 0 ? t[repeats / 2] : 0.0;
+    if (visible_out) *visible_out = v;
+    free(keys);
+    free(idx);
+    free(t);
+    return med;
+}
+
+/* ---------------------------------------------------------------------- */
+/* CPU baseline on all host cores: renderer.rs:372-396 with the transform  */
+/* split over threads and std's stable sort replaced by a parallel stable  */
+/* merge sort.  Same visible set and order as orc_cpu_depth_sort.          */
+/* ---------------------------------------------------------------------- */
+
+typedef struct {
+    const float* positions;
+    size_t lo, hi;
+    const float* vp;
+    float tol;
+    uint32_t* keys;  /* this thread's visible entries, in splat order */
+    uint32_t* idx;
+    size_t count;
+} orc_cull_job;
+
+static void* cull_range(void* arg) {
+    orc_cull_job* j = (orc_cull_job*)arg;
+    size_t v = 0;
+    for (size_t i = j->lo; i < j->hi; ++i) {
+        float p[4] = {j->positions[3 * i], j->positions[3 * i + 1], j->positions[3 * i + 2], 1.0f};
+        float h[4];
+        orc_mat4_transform(j->vp, p, h);
+        float r = 1.0f / h[3];
+        float c0 = h[0] * r, c1 = h[1] * r, c2 = h[2] * r;
+        if (fabsf(c0) < j->tol && fabsf(c1) < j->tol && fabsf(c2 - 0.5f) < 0.5f) {
+            j->keys[v] = f2u(c2);
+            j->idx[v] = (uint32_t)i;
+            ++v;
+        }
+    }
+    j->count = v;
+    return NULL;
+}
+
+/* Merge-path co-rank: how many of the first k outputs of the stable merge of
+ * a[0..m) and b[0..n) come from a (equal keys: a's first). */
+static size_t co_rank(size_t k, const uint32_t* a, size_t m, const uint32_t* b, size_t n) {
+    size_t i = k < m ? k : m, j = k - i;
+    size_t i_lo = k > n ? k - n : 0, j_lo = k > m ? k - m : 0;
+    for (;;) {
+        if (i > 0 && j < n && a[i - 1] > b[j]) {
+            size_t d = (i - i_lo + 1) / 2;
+            j_lo = j;
+            i -= d;
+            j += d;
+        } else if (j > 0 && i < m && b[j - 1] >= a[i]) {
+            size_t d = (j - j_lo + 1) / 2;
+            i_lo = i;
+            i += d;
+            j -= d;
+        } else {
+            return i;
+        }
+    }
+}
+
+typedef struct {
+    const uint32_t *ak, *av, *bk, *bv;
+    size_t m, n;
+    uint32_t *ok, *ov;
+    size_t out_lo, out_hi;   /* slice of the merged output this job writes */
+    uint32_t* sort_k;        /* or, when set: stable-sort this chunk in place */
+    uint32_t* sort_v;
+    uint32_t* scratch;
+    size_t sort_n;
+} orc_merge_job;
+
+static void* merge_slice(void* arg) {
+    orc_merge_job* j = (orc_merge_job*)arg;
+    if (j->sort_k) {
+        orc_stable_sort_pairs(j->sort_k, j->sort_v, j->sort_n, j->scratch);
+        return NULL;
+    }
+    size_t i = co_rank(j->out_lo, j->ak, j->m, j->bk, j->n);
+    size_t jj = j->out_lo - i;
+    for (size_t o = j->out_lo; o < j->out_hi; ++o) {
+        if (jj < j->n && (i >= j->m || j->bk[jj] < j->ak[i])) {
+            j->ok[o] = j->bk[jj];
+            j->ov[o] = j->bv[jj++];
+        } else {
+            j->ok[o] = j->ak[i];
+            j->ov[o] = j->av[i++];
+        }
+    }
+    return NULL;
+}
+
+static void run_jobs(void* (*fn)(void*), void* jobs, size_t stride, int count) {
+    pthread_t th[ORC_MAX_THREADS];
+    int started[ORC_MAX_THREADS];
+    for (int t = 0; t < count; ++t)
+        started[t] = pthread_create(&th[t], NULL, fn, (char*)jobs + (size_t)t * stride) == 0;
+    for (int t = 0; t < count; ++t) {
+        if (started[t]) pthread_join(th[t], NULL);
+        else fn((char*)jobs + (size_t)t * stride);  /* no thread to be had: run it here */
+    }
+}
+
+size_t orc_cpu_depth_sort_threads(const float* positions, size_t count, const float vp[16], float tol,
+                                  int threads, uint32_t* keys, uint32_t* indices) {
+    if (threads < 1) threads = 1;
+    if (threads > ORC_MAX_THREADS) threads = ORC_MAX_THREADS;
+    orc_cull_job cj[ORC_MAX_THREADS];
+    uint32_t* tk = (uint32_t*)malloc(2 * (count ? count : 1) * sizeof(uint32_t));
+    uint32_t* ti = tk + (count ? count : 1);
+    for (int t = 0; t < threads; ++t) {
+        cj[t].positions = positions;
+        cj[t].lo = count * (size_t)t / (size_t)threads;
+        cj[t].hi = count * (size_t)(t + 1) / (size_t)threads;
+        cj[t].vp = vp;
+        cj[t].tol = tol;
+        cj[t].keys = tk + cj[t].lo;
+        cj[t].idx = ti + cj[t].lo;
+    }
+    run_jobs(cull_range, cj, sizeof(orc_cull_job), threads);
+    /* compact the per-thread runs in splat order: filter_map's order */
+    size_t v = 0, run_lo[ORC_MAX_THREADS + 1];
+    for (int t = 0; t < threads; ++t) {
+        run_lo[t] = v;
+        memcpy(keys + v, cj[t].keys, cj[t].count * sizeof(uint32_t));
+        memcpy(indices + v, cj[t].idx, cj[t].count * sizeof(uint32_t));
+        v += cj[t].count;
+    }
+    run_lo[threads] = v;
+    /* each thread sorts one chunk, then chunks merge pairwise; every merge
+     * round splits its output evenly over the threads by merge path */
+    uint32_t* sk = (uint32_t*)malloc(2 * (v ? v : 1) * sizeof(uint32_t));
+    uint32_t* sv = sk + (v ? v : 1);
+    orc_merge_job mj[ORC_MAX_THREADS];
+    memset(mj, 0, sizeof(mj));
+    for (int t = 0; t < threads; ++t) {
+        mj[t].sort_k = keys + run_lo[t];
+        mj[t].sort_v = indices + run_lo[t];
+        mj[t].sort_n = run_lo[t + 1] - run_lo[t];
+        mj[t].scratch = tk + 2 * run_lo[t];  /* 2 * chunk words, inside the transform buffers */
+    }
+    run_jobs(merge_slice, mj, sizeof(orc_merge_job), threads);
+    uint32_t *ck = keys, *cv = indices, *nk = sk, *nv = sv;
+    size_t bounds[ORC_MAX_THREADS + 1];
+    int runs = threads;
+    for (int t = 0; t <= threads; ++t) bounds[t] = run_lo[t];
+    while (runs > 1) {
+        int jobs = 0;
+        size_t nb[ORC_MAX_THREADS + 1];
+        int nr = 0;
+        for (int r = 0; r < runs; r += 2) {
+            size_t a0 = bounds[r], a1 = bounds[r + 1];
+            size_t b1 = r + 1 < runs ? bounds[r + 2] : a1;
+            size_t total = b1 - a0;
+            int share = threads / ((runs + 1) / 2);
+            if (share < 1) share = 1;
+            for (int s = 0; s < share; ++s) {
+                orc_merge_job* j = &mj[jobs++];
+                memset(j, 0, sizeof(*j));
+                j->ak = ck + a0; j->av = cv + a0; j->m = a1 - a0;
+                j->bk = ck + a1; j->bv = cv + a1; j->n = b1 - a1;
+                j->ok = nk + a0; j->ov = nv + a0;
+                j->out_lo = total * (size_t)s / (size_t)share;
+                j->out_hi = total * (size_t)(s + 1) / (size_t)share;
+            }
+            nb[nr++] = a0;
+        }
+        nb[nr] = v;
+        run_jobs(merge_slice, mj, sizeof(orc_merge_job), jobs);
+        uint32_t* t;
+        t = ck; ck = nk; nk = t;
+        t = cv; cv = nv; nv = t;
+        runs = nr;
+        for (int r = 0; r <= nr; ++r) bounds[r] = nb[r];
+    }
+    if (ck != keys) {
+        memcpy(keys, ck, v * sizeof(uint32_t));
+        memcpy(indices, cv, v * sizeof(uint32_t));
+    }
+    free(sk);
+    free(tk);
+    return v;
+}
+
+double orc_time_cpu_depth_sort_threads(const float* positions, size_t count, const float vp[16], float tol,
+                                       int threads, int repeats, size_t* visible_out) {
+    uint32_t* keys = (uint32_t*)malloc((count ? count : 1) * sizeof(uint32_t));
+    uint32_t* idx = (uint32_t*)malloc((count ? count : 1) * sizeof(uint32_t));
+    double* t = (double*)malloc((size_t)(repeats > 0 ? repeats : 1) * sizeof(double));
+    size_t v = 0;
+    for (int w = 0; w < 2; ++w) v = orc_cpu_depth_sort_threads(positions, count, vp, tol, threads, keys, idx);
+    for (int i = 0; i < repeats; ++i) {
+        double t0 = now_s();
+        v = orc_cpu_depth_sort_threads(positions, count, vp, tol, threads, keys, idx);
         t[i] = now_s() - t0;
     }
     qsort(t, (size_t)repeats, sizeof(double), cmp_double);

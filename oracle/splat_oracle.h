@@ -148,6 +148,17 @@ size_t orc_render_frame(const orc_splat* splats, size_t count, const orc_uniform
 double orc_time_cpu_depth_sort(const float* positions, size_t count, const float vp[16],
                                float tol, int repeats, size_t* visible_out);
 
+/* The same path on `threads` host threads (at most ORC_MAX_THREADS): the
+ * transform split over contiguous splat ranges, then a parallel stable merge
+ * sort (per-thread merge sort of one chunk each, then merge rounds split by
+ * merge path).  Output identical to orc_cpu_depth_sort. */
+#define ORC_MAX_THREADS 64
+size_t orc_cpu_depth_sort_threads(const float* positions, size_t count, const float vp[16], float tol,
+                                  int threads, uint32_t* keys, uint32_t* indices);
+/* Median seconds of `repeats` runs after two warm-up runs. */
+double orc_time_cpu_depth_sort_threads(const float* positions, size_t count, const float vp[16], float tol,
+                                       int threads, int repeats, size_t* visible_out);
+
 #ifdef __cplusplus
 }
 #endif

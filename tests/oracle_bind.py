@@ -68,6 +68,11 @@ def lib():
         l.orc_render_frame.restype = C.c_size_t
         l.orc_time_cpu_depth_sort.argtypes = [P, C.c_size_t, P, C.c_float, C.c_int, C.POINTER(C.c_size_t)]
         l.orc_time_cpu_depth_sort.restype = C.c_double
+        l.orc_cpu_depth_sort_threads.argtypes = [P, C.c_size_t, P, C.c_float, C.c_int, P, P]
+        l.orc_cpu_depth_sort_threads.restype = C.c_size_t
+        l.orc_time_cpu_depth_sort_threads.argtypes = [P, C.c_size_t, P, C.c_float, C.c_int, C.c_int,
+                                                      C.POINTER(C.c_size_t)]
+        l.orc_time_cpu_depth_sort_threads.restype = C.c_double
         _lib = l
     return _lib
 
@@ -109,6 +114,18 @@ def cpu_depth_sort(positions: np.ndarray, vp: np.ndarray, tol: float):
     idx = np.empty(max(n, 1), np.uint32)
     vpa = np.ascontiguousarray(vp, dtype=np.float32).reshape(16)
     v = lib().orc_cpu_depth_sort(ptr(pos), n, ptr(vpa), float(tol), ptr(keys), ptr(idx))
+    return keys[:v].copy(), idx[:v].copy()
+
+
+def cpu_depth_sort_threads(positions: np.ndarray, vp: np.ndarray, tol: float, threads: int):
+    """renderer.rs:372-396 on `threads` host threads (parallel transform and
+    stable merge sort); the same entries as cpu_depth_sort."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    n = len(pos)
+    keys = np.empty(max(n, 1), np.uint32)
+    idx = np.empty(max(n, 1), np.uint32)
+    vpa = np.ascontiguousarray(vp, dtype=np.float32).reshape(16)
+    v = lib().orc_cpu_depth_sort_threads(ptr(pos), n, ptr(vpa), float(tol), int(threads), ptr(keys), ptr(idx))
     return keys[:v].copy(), idx[:v].copy()
 
 

@@ -194,3 +194,19 @@ def test_vertex_stage_predicate_divides_by_w_plus_1e7():
     planted = visible(h[3])
     assert np.array_equal(got, want)
     assert len(np.setxor1d(got, planted)) >= 10, len(np.setxor1d(got, planted))
+
+
+@pytest.mark.parametrize("threads", [1, 2, 3, 8, 16])
+def test_all_core_depth_sort_equals_one_core(threads):
+    """The all-cores CPU baseline leg (parallel transform, parallel stable
+    merge sort) returns exactly the one-core fallback's entries, on
+    tie-heavy input and on the degenerate sizes."""
+    w = Workload("tight", 300_000, seed=3).all()
+    cam, view = sphere_pose(7, radius=8.0)
+    vp = ob.vp_of(ob.uniforms(cam, view, 1920, 1080))
+    for n in (0, 1, 5, 4097, 300_000):
+        pos = w["center"][:n]
+        k1, i1 = ob.cpu_depth_sort(pos, vp, 1.1)
+        kt, it = ob.cpu_depth_sort_threads(pos, vp, 1.1, threads)
+        assert np.array_equal(k1, kt) and np.array_equal(i1, it), n
+    assert len(i1) > 1000 and len(np.unique(k1)) < len(k1)  # ties present
