@@ -304,6 +304,8 @@ def main() -> int:
                     help="batches in flight: the batched loop alternates this many renderers, each with its own "
                          "frames, so one batch's raster overlaps the next one's sort and projection")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrivals per second")
+    ap.add_argument("--serving-rate-high", type=float, default=1200.0,
+                    help="a second Poisson rate, high enough that requests queue and batches form (0 skips it)")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="0 skips the serving measurement")
     ap.add_argument("--serving-sweep", type=str, default="",
                     help="comma-separated request rates: run only the serving measurement at each and print them")
@@ -508,15 +510,19 @@ def main() -> int:
     sort_i = STAGES.index("depth_sort")
     sort_gkeys = V / (stage_ms[sort_i] * 1e-3) / 1e9 if stage_ms[sort_i] > 0 else None
 
-    serve = None
+    serve = serve_high = None
     if args.serving_seconds > 0:
         r.close()
         scene.close()
         server, scenes, setup = serving_setup(dev, sa)
         serve = serving(server, scenes, setup, args.serving_rate, args.serving_seconds)
-        server.close()
         log(f"serving at {args.serving_rate}/s: p50 {serve['p50_ms']} ms, p99 {serve['p99_ms']} ms, "
             f"{serve['frames_per_s']} frames/s")
+        if args.serving_rate_high > 0:
+            serve_high = serving(server, scenes, setup, args.serving_rate_high, args.serving_seconds)
+            log(f"serving at {args.serving_rate_high}/s: p50 {serve_high['p50_ms']} ms, p99 {serve_high['p99_ms']} ms, "
+                f"{serve_high['frames_per_s']} frames/s, mean batch {serve_high['mean_batch_views']}")
+        server.close()
 
     cpu = None
     if not args.no_cpu:
@@ -556,6 +562,7 @@ def main() -> int:
                                    "p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
                                    "kind": "single plain frame, submit to ready, back to back"},
         "serving": serve,
+        "serving_high_rate": serve_high,
         "visible_per_view": int(V),
         "pairs_per_view": int(K),
         "stages": per_stage,
