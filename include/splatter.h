@@ -174,6 +174,23 @@ int spl_render_views(spl_renderer* renderer, spl_scene* scene, uint32_t view_cou
                      spl_frame* const* frames, const float* camera_matrices,
                      const float* view_matrices);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  spl_render_frame for one view with no renderer: the
+ * whole workspace is a fixed 48 MiB scratch array that the library holds on
+ * each device it runs on (looked up once, when the device opens), so the
+ * call allocates nothing.  Scenes up to 131072 splats and frames up to
+ * 3840 x 2160; larger ones fail with SPL_ERROR_CAPACITY.  config is checked
+ * as spl_renderer_create checks it; max_views must be 0 or 1, and the
+ * (splat, screen tile) pair capacity is what the splats and the frame leave
+ * of the scratch (max_splat_count and tile_pairs_per_splat are not used).
+ * Asynchronous on the device's stream like spl_render_frame; the image is
+ * the same.  Every call on one device shares the scratch array and keeps
+ * what the last call left there: calls that overlap on one device (from
+ * several threads, or with the device stream's work reordered) race. */
+int spl_render_frame_static(spl_device* device, const spl_configuration* config, spl_scene* scene,
+                            spl_frame* frame, const float camera_matrix[16], const float view_matrix[16]);
+
 /* ---- serving form (no counterpart in the reference) ---------------------- */
 /* A request loop over resident scenes: each request is one view of one scene
  * at its own resolution.  Up to max_in_flight batches run at once, each on a

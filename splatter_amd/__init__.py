@@ -127,6 +127,16 @@ class Device:
     def synchronize(self) -> None:
         check(lib().spl_synchronize(self._h))
 
+    def render_frame_static(self, config: "Configuration", frame: "Frame", camera_matrix, view_matrix,
+                            scene: "Scene") -> None:
+        """Renderer.render_frame without a renderer and without allocating:
+        the workspace is the library's fixed scratch array on this device
+        (scenes up to 131072 splats, frames up to 3840x2160).  Calls on one
+        device share that array and must not overlap."""
+        c = config.to_c()
+        check(lib().spl_render_frame_static(self._h, C.byref(c), scene.handle, frame.handle,
+                                            _mat16(camera_matrix), _mat16(view_matrix)))
+
     def close(self) -> None:
         if self._h:
             lib().spl_device_close(self._h)

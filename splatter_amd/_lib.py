@@ -111,6 +111,7 @@ SIGNATURES = [
     ("spl_frame_read", C.c_int, [_P, _P, C.c_uint64]),
     ("spl_render_frame", C.c_int, [_P, _P, _P, _F16, _F16]),
     ("spl_render_views", C.c_int, [_P, _P, C.c_uint32, _PP, _F16, _F16]),
+    ("spl_render_frame_static", C.c_int, [_P, C.POINTER(SplConfiguration), _P, _P, _F16, _F16]),
     ("spl_server_configuration_default", None, [C.POINTER(SplServerConfiguration)]),
     ("spl_server_create", C.c_int, [_P, C.POINTER(SplServerConfiguration), _PP]),
     ("spl_server_destroy", C.c_int, [_P]),

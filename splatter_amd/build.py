@@ -23,7 +23,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsplatter_amd.so")
-SOURCES = ["cull.hip", "sort.hip", "project.hip", "raster.hip", "api.cpp", "renderer.cpp", "server.cpp"]
+SOURCES = ["cull.hip", "sort.hip", "project.hip", "raster.hip", "static_frame.hip", "api.cpp", "renderer.cpp",
+           "server.cpp"]
 ARCH = os.environ.get("SPLATTER_OFFLOAD_ARCH", "gfx950")
 
 

@@ -93,6 +93,13 @@ int spl_device_open(int ordinal, spl_device** out) {
   }
   (void)hipDeviceGetAttribute(&d->cu_count, hipDeviceAttributeMultiprocessorCount, ordinal);
   if (d->cu_count <= 0) d->cu_count = 256;
+  const int rc = spl::static_scratch_setup(d);  // with the device current
+  if (rc) {
+    for (int j = 0; j < kPoolStreams; ++j) (void)hipStreamDestroy(d->pool[j]);
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return rc;
+  }
   *out = d;
   return SPL_OK;
 }

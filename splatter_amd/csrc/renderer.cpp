@@ -11,6 +11,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "spl_internal.h"
@@ -622,15 +623,8 @@ int draw_order(spl_renderer* r, std::vector<uint32_t>& order, std::vector<uint32
   return SPL_OK;
 }
 
-}  // namespace
-
-namespace spl {
-
-int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stream, uint32_t presize_tiles,
-                    spl_renderer** out, uint64_t* dry_bytes) {
-  if (!dev || !cfg || (!out && !dry_bytes)) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_create: null argument");
-  if (out) *out = nullptr;
-  const bool dry = dry_bytes != nullptr;
+// The checks of spl_renderer_create on a configuration.
+int validate_config(const spl_configuration* cfg) {
   if (cfg->spherical_harmonics_order > 3)
     return fail(SPL_ERROR_INVALID_ARGUMENT, "spherical_harmonics_order must be 0..=3");
   if (cfg->radix_bits_per_digit != 8 && cfg->radix_bits_per_digit != 10)
@@ -650,20 +644,18 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   if (cfg->raster_batch != 0 && cfg->raster_batch != 128 && cfg->raster_batch != 256)
     return fail(SPL_ERROR_UNSUPPORTED, "raster_batch must be 128 or 256");
   if (cfg->raster_longest_first > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_longest_first must be 0 or 1");
-  HIP_TRY(hipSetDevice(dev->ordinal));
-  auto* r = new (std::nothrow) spl_renderer();
-  if (!r) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: host allocation");
-  r->dev = dev;
+  return SPL_OK;
+}
+
+// Defaults, digit passes, sort block shape and the per-view word counts of
+// the workspace for `cap` splats and `pair_cap` (splat, tile) pairs
+// (0: tile_pairs_per_splat per splat).
+void size_workspace(spl_renderer* r, const spl_configuration* cfg, uint64_t cap, uint64_t pair_cap) {
   r->cfg = *cfg;
-  r->stream = dev->stream;
-  if (own_stream && !dry) {
-    r->stream = acquire_stream(dev);
-    r->own_stream = true;
-  }
   if (r->cfg.tile_pairs_per_splat == 0) r->cfg.tile_pairs_per_splat = 4;
   if (r->cfg.max_views == 0) r->cfg.max_views = 1;
   r->max_views = r->cfg.max_views;
-  r->cap = cfg->max_splat_count;
+  r->cap = cap;
   r->depth_bits = (int)cfg->radix_bits_per_digit;
   r->depth_passes = (30 + r->depth_bits - 1) / r->depth_bits;
   if (r->cfg.sort_tile_keys == 0) r->cfg.sort_tile_keys = 4096;
@@ -672,12 +664,36 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
                     : r->cfg.sort_tile_keys == 8192 ? kSortThreadsWide : kSortThreads;
   const int radix = 1 << r->depth_bits;
   // a multiple of 4: the tile-range pass reads the pair keys as uint4
-  r->pair_cap = std::min<uint64_t>(r->cap * (uint64_t)r->cfg.tile_pairs_per_splat, (1ull << 30) - (1ull << 20)) & ~3ull;
+  if (pair_cap == 0) pair_cap = r->cap * (uint64_t)r->cfg.tile_pairs_per_splat;
+  r->pair_cap = std::min<uint64_t>(pair_cap, (1ull << 30) - (1ull << 20)) & ~3ull;
   if (r->pair_cap == 0) r->pair_cap = 4;
   r->cull_tiles = div_up(r->cap, kCullTile);
   r->sort_words = (size_t)div_up(r->cap, kSortThreads * kSortItems) * radix;
   r->bin_words = div_up(r->cap, kBinThreads * kBinItems);
   r->pair_words = (size_t)div_up(r->pair_cap, kSortThreads * kSortItems) * 256;
+}
+
+}  // namespace
+
+namespace spl {
+
+int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stream, uint32_t presize_tiles,
+                    spl_renderer** out, uint64_t* dry_bytes) {
+  if (!dev || !cfg || (!out && !dry_bytes)) return fail(SPL_ERROR_INVALID_ARGUMENT, "spl_renderer_create: null argument");
+  if (out) *out = nullptr;
+  const bool dry = dry_bytes != nullptr;
+  int rc0 = validate_config(cfg);
+  if (rc0) return rc0;
+  HIP_TRY(hipSetDevice(dev->ordinal));
+  auto* r = new (std::nothrow) spl_renderer();
+  if (!r) return fail(SPL_ERROR_OUT_OF_MEMORY, "spl_renderer_create: host allocation");
+  r->dev = dev;
+  r->stream = dev->stream;
+  if (own_stream && !dry) {
+    r->stream = acquire_stream(dev);
+    r->own_stream = true;
+  }
+  size_workspace(r, cfg, cfg->max_splat_count, 0);
   const size_t V = r->max_views;
   hipError_t e = hipSuccess;
   auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
@@ -747,6 +763,82 @@ int renderer_create(spl_device* dev, const spl_configuration* cfg, bool own_stre
   }
   device_retain(dev);
   *out = r;
+  return SPL_OK;
+}
+
+int render_frame_scratch(spl_device* dev, const spl_configuration* cfg, spl_scene* s, spl_frame* f,
+                         const float cam[16], const float view[16], uint8_t* scratch, uint64_t bytes) {
+  int rc = validate_config(cfg);
+  if (rc) return rc;
+  spl_renderer r;  // host-side description only: every buffer below points into `scratch`
+  r.dev = dev;
+  r.stream = dev->stream;
+  const uint64_t cap = std::max<uint64_t>(s->count, 1);
+  Uniforms u;
+  frame_uniforms(*cfg, cam, view, f->width, f->height, u);
+  const uint32_t tiles = u.tiles_x * u.tiles_y;
+  uint64_t zero_bytes = 0;
+  // Carves the workspace at 256-byte alignment; the words that must start
+  // at zero (counters, status words, histograms, tile ranges) come first, so
+  // one fill clears them.  Returns the bytes used.
+  auto carve = [&](uint8_t* base) -> uint64_t {
+    uint64_t off = 0;
+    auto take = [&](auto** p, uint64_t count) {
+      *p = reinterpret_cast<std::remove_reference_t<decltype(*p)>>(base + off);
+      off += (std::max<uint64_t>(count, 1) * sizeof(**p) + 255) & ~255ull;
+    };
+    take(&r.counters, kCounterWords);
+    take(&r.cull_status, r.cull_tiles);
+    take(&r.sort_status, r.sort_words * kMaxDepthPasses);
+    take(&r.bin_status, r.bin_words);
+    take(&r.pair_status, r.pair_words * kMaxPairPasses);
+    take(&r.depth_hist, (uint64_t)kMaxDepthPasses * 1024);
+    take(&r.pair_hist, (uint64_t)kMaxPairPasses * 256);
+    take(&r.ranges, tiles);
+    zero_bytes = off;
+    take(&r.d_uniforms, 1);
+    take(&r.d_pixels, 1);
+    take(&r.tile_prefix, r.cull_tiles);
+    take(&r.vis, r.cap);
+    for (int i = 0; i < 2; ++i) {
+      take(&r.keys[i], r.cap);
+      take(&r.vals[i], r.cap);
+    }
+    take(&r.compact_idx, r.cap);
+    take(&r.records, r.cap * kRecordFloat4);
+    take(&r.boxes, r.cap);
+    take(&r.tile_order, tiles);
+    for (int i = 0; i < 2; ++i) {
+      take(&r.pair_keys[i], r.pair_cap);
+      take(&r.pair_vals[i], r.pair_cap);
+    }
+    return off;
+  };
+  // The (splat, tile) pairs take whatever the rest of the scratch holds.
+  spl_configuration one = *cfg;
+  one.max_views = 1;
+  size_workspace(&r, &one, cap, 4);
+  const uint64_t fixed = carve(scratch);
+  if (fixed >= bytes) return fail(SPL_ERROR_CAPACITY, "spl_render_frame_static: scene and frame exceed the scratch");
+  uint64_t pairs = (bytes - fixed) / 17;
+  for (;;) {
+    size_workspace(&r, &one, cap, pairs);
+    if (carve(scratch) <= bytes) break;
+    pairs -= pairs / 64 + 4;
+    if (pairs < 4) return fail(SPL_ERROR_CAPACITY, "spl_render_frame_static: scene and frame exceed the scratch");
+  }
+  r.ranges_cap = tiles;
+  launch_static_setup(u, f->pixels, r.d_uniforms, r.d_pixels, reinterpret_cast<uint32_t*>(scratch),
+                      zero_bytes / 4, r.stream);
+  BatchShape b;
+  b.scene = s;
+  b.scene_uid = s->uid;
+  b.n = s->count;
+  b.views = 1;
+  b.max_tiles = tiles;
+  b.pair_passes = pair_passes_for(tiles);
+  launch_sequence(&r, b, r.stream, nullptr);
+  HIP_TRY(hipGetLastError());
   return SPL_OK;
 }
 

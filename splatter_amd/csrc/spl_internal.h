@@ -192,4 +192,15 @@ void launch_ranges(const RangesLaunch& a, hipStream_t s);
 void launch_tile_order(const OrderLaunch& a, hipStream_t s);
 void launch_raster(const RasterLaunch& a, hipStream_t s);
 
+// spl_render_frame_static: the scratch array's size and the largest scene and
+// frame it takes.
+constexpr uint64_t kStaticScratchBytes = 48ull << 20;
+constexpr uint32_t kStaticMaxSplats = 1u << 17;
+constexpr uint32_t kStaticMaxWidth = 3840, kStaticMaxHeight = 2160;
+// Writes the frame's uniforms and pixel pointer and zeroes `zero_words`
+// words at `zero` (the bookkeeping that starts every frame at zero), with
+// one kernel.
+void launch_static_setup(const Uniforms& u, uint32_t* pixels, Uniforms* d_uniforms, uint32_t** d_pixels,
+                         uint32_t* zero, size_t zero_words, hipStream_t s);
+
 }  // namespace spl

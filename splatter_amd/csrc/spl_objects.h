@@ -34,6 +34,15 @@ int renderer_counter_words(const spl_renderer* r, uint32_t view, const uint32_t*
 // Scene::new without a renderer: `count` splats, zeroed, on `dev`.
 int scene_create(spl_device* dev, uint64_t count, spl_scene** out);
 
+// One frame of spl_render_frame's pipeline with every workspace buffer carved
+// from `scratch` (`bytes` of device memory): no allocation.  The (splat,
+// tile) pairs get what the splats and the frame leave of it.
+int render_frame_scratch(spl_device* dev, const spl_configuration* cfg, spl_scene* s, spl_frame* f,
+                         const float cam[16], const float view[16], uint8_t* scratch, uint64_t bytes);
+// Looks up this library's scratch array on device `dev` (current) once and
+// keeps its address in dev->static_scratch.
+int static_scratch_setup(spl_device* dev);
+
 // Every renderer, scene, frame and server holds a reference to its device,
 // as does the handle spl_device_open returns: the device (and its stream)
 // goes with the last of them, whatever order they are destroyed in.
@@ -77,6 +86,7 @@ struct spl_device {
   int cu_count = 0;
   std::atomic<int> refs{1};  // the opener's handle + one per object created on the device
   hipStream_t pool[kPoolStreams] = {};
+  void* static_scratch = nullptr;  // this device's copy of the library's scratch array (spl_render_frame_static)
   int pool_users[kPoolStreams] = {};
   std::mutex pool_mu;
 };
