@@ -96,7 +96,10 @@ typedef struct spl_configuration {
     uint32_t raster_mode;                 /* 0: each wave's lanes test every entry of its 8x8 quadrant;
                                              1: each lane walks the entries covering its pixel;
                                              2: four lane groups step through the lists of their 4x4
-                                             sub-blocks side by side (all the same image) */
+                                             sub-blocks side by side;
+                                             3: each lane marks the entries covering its pixel 64 list
+                                             slots at a time, then walks them on its own (all the same
+                                             image) */
     uint32_t raster_batch;                /* entries a raster block stages per batch: 128 or 256
                                              (batches of two or more views; one view uses 256) */
     uint32_t raster_longest_first;        /* 1: tiles with the longest lists start first (same image;

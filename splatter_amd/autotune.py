@@ -20,7 +20,7 @@ import numpy as np
 
 RADIX_BITS = (8, 10)
 SORT_TILE_KEYS = (4096, 8192, 16384)
-RASTER_MODE = (0, 1, 2)
+RASTER_MODE = (0, 1, 2, 3)
 RASTER_BATCH = (128, 256)
 RASTER_LONGEST_FIRST = (0, 1)
 BATCH_VIEWS = (8, 16, 32)

@@ -268,10 +268,10 @@ void launch_tile_order(const OrderLaunch& a, hipStream_t s) {
 // (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
 // stops blending, and the block stops when all four have.
 template <bool UNALIGNED, bool F32, int MODE, int B>
-__global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
+__global__ __launch_bounds__(256, MODE == 3 ? 8 : 1) void k_raster(RasterLaunch a) {
   static_assert(B == 128 || B == 256, "entries staged per batch");
   constexpr int U = B / 64;  // list slots per step of the entry-major walks
-  constexpr bool WALK = MODE == 1, SUBS = MODE == 2;
+  constexpr bool WALK = MODE == 1, SUBS = MODE == 2, LANES = MODE == 3;
   // per staged entry: [0] i00 i01 i10 i11, [1] tx ty half alpha, [2] r g b -
   // (one array, so an entry's three rows share one LDS address)
   __shared__ float4 s_ent[2][3][B];
@@ -549,6 +549,70 @@ __global__ __launch_bounds__(256) void k_raster(RasterLaunch a) {
             if (i + k < mine && covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k]);
           }
         }
+      } else if constexpr (LANES) {
+        // per-lane walk: for 32 list slots at a time every lane first marks,
+        // slot after slot, the entries whose rectangle covers its pixel
+        // (entry-uniform LDS reads, one coverage bit per slot), then walks
+        // only those, in list order, each lane on its own: the wave takes as
+        // many steps as its busiest pixel has covering entries, not as many
+        // as the list has.  The fragment arithmetic is the entry-major
+        // walk's, in the same order per pixel, so the pixels are the same.
+        for (uint32_t c = 0; c < n; c += 32) {
+          const uint32_t cnt = min(32u, n - c);
+          uint32_t mask = 0;
+          for (uint32_t k = 0; k < cnt; k += 4) {
+            // past cnt the slots may be stale or uninitialised: masked into
+            // [0, B) they index s_ent, and their bits are cleared below
+            const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][c + k]);
+            uint32_t nib = 0;
+            if constexpr (unaligned) {
+              // the half-extent is margin * S for every entry: only the map
+              // and the translation are read
+              float4 m[4];
+              float2 t[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const uint32_t j = (js >> (8 * q)) & (uint32_t)(B - 1);
+                m[q] = s_ent[buf][0][j];
+                t[q] = *reinterpret_cast<const float2*>(&s_ent[buf][1][j]);
+              }
+              const float h = margin * kGaussScale;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                float u, w2;
+                nib |= (covers(m[q], make_float4(t[q].x, t[q].y, h, 0.0f), u, w2) ? 1u : 0u) << q;
+              }
+            } else {
+              float4 m[4], t[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const uint32_t j = (js >> (8 * q)) & (uint32_t)(B - 1);
+                m[q] = s_ent[buf][0][j];
+                t[q] = s_ent[buf][1][j];
+              }
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                float u, w2;
+                nib |= (covers(m[q], t[q], u, w2) ? 1u : 0u) << q;
+              }
+            }
+            mask |= nib << k;
+          }
+          if (cnt < 32) mask &= (1u << cnt) - 1u;
+          if (tr == 0.0f) mask = 0u;
+          while (mask) {
+            const uint32_t k = (uint32_t)__builtin_ctz(mask);
+            mask &= mask - 1u;
+            // each lane reads slot k's batch index from the list itself (a
+            // cross-lane read would see 0 from lanes that have left the loop)
+            const uint32_t j = (uint32_t)s_list[w][c + k] & (uint32_t)(B - 1);
+            const float4 t = s_ent[buf][1][j];
+            float u, w2;
+            (void)covers(s_ent[buf][0][j], t, u, w2);
+            shade(u, w2, t.w, s_ent[buf][2][j]);
+            if (tr == 0.0f) mask = 0u;
+          }
+        }
       } else if constexpr (!WALK) {
         // entry-major: the wave steps through its list, every lane testing
         // its pixel; four list slots per step so their LDS reads go together
@@ -642,7 +706,8 @@ void launch_raster_as(bool small, dim3 grid, dim3 block, hipStream_t s, const Ra
 
 template <bool UNALIGNED, bool F32>
 void launch_raster_mode(bool small, dim3 grid, dim3 block, hipStream_t s, const RasterLaunch& a) {
-  if (a.walk == 2) launch_raster_as<UNALIGNED, F32, 2>(small, grid, block, s, a);
+  if (a.walk == 3) launch_raster_as<UNALIGNED, F32, 3>(small, grid, block, s, a);
+  else if (a.walk == 2) launch_raster_as<UNALIGNED, F32, 2>(small, grid, block, s, a);
   else if (a.walk == 1) launch_raster_as<UNALIGNED, F32, 1>(small, grid, block, s, a);
   else launch_raster_as<UNALIGNED, F32, 0>(small, grid, block, s, a);
 }

@@ -640,7 +640,7 @@ int validate_config(const spl_configuration* cfg) {
   if (cfg->sort_tile_keys != 0 && cfg->sort_tile_keys != 4096 && cfg->sort_tile_keys != 8192 &&
       cfg->sort_tile_keys != 16384)
     return fail(SPL_ERROR_UNSUPPORTED, "sort_tile_keys must be 4096, 8192 or 16384");
-  if (cfg->raster_mode > 2) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_mode must be 0, 1 or 2");
+  if (cfg->raster_mode > 3) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_mode must be 0, 1, 2 or 3");
   if (cfg->raster_batch != 0 && cfg->raster_batch != 128 && cfg->raster_batch != 256)
     return fail(SPL_ERROR_UNSUPPORTED, "raster_batch must be 128 or 256");
   if (cfg->raster_longest_first > 1) return fail(SPL_ERROR_INVALID_ARGUMENT, "raster_longest_first must be 0 or 1");

@@ -175,7 +175,8 @@ struct RasterLaunch {
   uint32_t* counters;
   int use_unaligned_rectangles;
   int blend_f32;
-  int walk;                  // raster mode: 0 entry-major per quadrant, 1 per-pixel walk, 2 per 4x4 sub-block
+  int walk;                  // raster mode: 0 entry-major per quadrant, 1 per-pixel walk, 2 per 4x4 sub-block,
+                             // 3 per-lane walk over coverage masks
   int batch_entries;         // entries staged in LDS per batch: 128 or 256
   int longest_first;         // take the tiles in the order k_tile_order wrote
   int bgra;

@@ -113,7 +113,7 @@ def test_depth_sorting_none_draws_in_index_order():
 
 @pytest.mark.parametrize("bits,tile,mode,batch,longest", [
     (10, 4096, 0, 256, 0), (8, 8192, 0, 128, 1), (10, 8192, 1, 128, 0), (8, 4096, 1, 256, 1),
-    (8, 16384, 2, 128, 0), (10, 16384, 2, 256, 1)])
+    (8, 16384, 2, 128, 0), (10, 16384, 2, 256, 1), (10, 16384, 3, 128, 0), (8, 8192, 3, 256, 1)])
 def test_autotuner_options(bits, tile, mode, batch, longest):
     # every option the autotuner may pick gives the same bit-exact order and image
     say(f"radix bits {bits}, sort tile {tile}, raster mode {mode}, batch {batch}, longest first {longest}")
@@ -124,7 +124,8 @@ def test_autotuner_options(bits, tile, mode, batch, longest):
 
 
 @pytest.mark.parametrize("mode,unaligned,blend", [(1, True, sa.BlendMode.F32), (1, False, sa.BlendMode.UNORM8),
-                                                  (2, True, sa.BlendMode.F32), (2, False, sa.BlendMode.UNORM8)])
+                                                  (2, True, sa.BlendMode.F32), (2, False, sa.BlendMode.UNORM8),
+                                                  (3, True, sa.BlendMode.F32), (3, False, sa.BlendMode.UNORM8)])
 def test_raster_modes_other_rectangles_and_blends(mode, unaligned, blend):
     case = Case(Workload("clustered", 300_000, seed=8),
                 config(raster_mode=mode, use_unaligned_rectangles=unaligned, blend_mode=blend), 640, 480,
