@@ -71,7 +71,7 @@ class Configuration:
     max_views: int = 1
     use_hip_graphs: bool = True
     sort_tile_keys: int = 4096
-    raster_mode: int = 0
+    raster_mode: int = 3             # per-lane walk over coverage masks (every mode renders the same image)
     raster_batch: int = 128          # batches of two or more views; one view rasters with 256
     raster_longest_first: int = 0    # batches of two or more views; one view always starts longest tiles first
 

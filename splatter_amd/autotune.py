@@ -1,7 +1,7 @@
 """Autotuner for the batched form: picks the radix digit width, the radix-sort
 tile (keys per tile, i.e. the sort block's lane count), the rasteriser's
-batch (entries a tile block stages per round), walk (entry-major or
-per-pixel) and tile order, and the batch width (views per launch sequence)
+batch (entries a tile block stages per round), walk (entry-major, per-pixel,
+per 4x4 sub-block or per-lane over coverage masks) and tile order, and the batch width (views per launch sequence)
 for one scene, resolution and pose set by timing the candidates on the
 device, and can keep the answer in a JSON cache.
 
@@ -170,7 +170,7 @@ TIE = 0.03
 
 
 def autotune_staged(sa, device, base, scene_loader, poses, width: int, height: int, views: int, steps: int = 3,
-                    log=None, walk=(0,), in_flight: int = 1, tie: float = TIE) -> TuneResult:
+                    log=None, walk=(3, 0), in_flight: int = 1, tie: float = TIE) -> TuneResult:
     """Coordinate search: the sort's digit width x tile with the raster at
     its defaults, then the raster's batch x tile order (x walk) with the best
     sort, at a fixed batch width and number of batches in flight.  Each stage
@@ -191,7 +191,7 @@ def autotune_staged(sa, device, base, scene_loader, poses, width: int, height: i
                 pick = (c, fps)
         return pick
 
-    start = Candidate(8, 4096, views, 0, base.raster_batch, 0)
+    start = Candidate(8, 4096, views, walk[0], base.raster_batch, 0)
     c, _ = run([start.replace(radix_bits_per_digit=r, sort_tile_keys=t)
                 for t in sorted(SORT_TILE_KEYS, reverse=True) for r in sorted(RADIX_BITS, reverse=True)])
     c, fps = run([c.replace(raster_batch=b, raster_longest_first=o, raster_mode=w)

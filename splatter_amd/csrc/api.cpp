@@ -64,6 +64,7 @@ void spl_configuration_default(spl_configuration* c) {
   c->max_views = 1;
   c->use_hip_graphs = 1;
   c->sort_tile_keys = 4096;
+  c->raster_mode = 3;
   c->raster_batch = 128;
 }
 

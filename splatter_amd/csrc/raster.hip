@@ -550,51 +550,45 @@ __global__ __launch_bounds__(256, MODE == 3 ? 8 : 1) void k_raster(RasterLaunch 
           }
         }
       } else if constexpr (LANES) {
-        // per-lane walk: for 32 list slots at a time every lane first marks,
-        // slot after slot, the entries whose rectangle covers its pixel
-        // (entry-uniform LDS reads, one coverage bit per slot), then walks
-        // only those, in list order, each lane on its own: the wave takes as
-        // many steps as its busiest pixel has covering entries, not as many
-        // as the list has.  The fragment arithmetic is the entry-major
-        // walk's, in the same order per pixel, so the pixels are the same.
+        // per-lane walk: for 32 list slots at a time the wave first copies
+        // the slots' entries into a compact run (its 32 entries of the
+        // other staging buffer, free until the next batch is staged), then
+        // every lane marks, slot after slot, the entries whose rectangle
+        // covers its pixel (entry-uniform LDS reads, one coverage bit per
+        // slot), and walks only those, in list order, each lane on its own:
+        // the wave takes as many steps as its busiest pixel has covering
+        // entries, not as many as the list has.  The fragment arithmetic is
+        // the entry-major walk's, in the same order per pixel, so the pixels
+        // are the same.
+        float4* const cm = &s_ent[buf ^ 1][0][32 * w];
+        float4* const ct = &s_ent[buf ^ 1][1][32 * w];
+        float4* const cc = &s_ent[buf ^ 1][2][32 * w];
         for (uint32_t c = 0; c < n; c += 32) {
           const uint32_t cnt = min(32u, n - c);
+          if (lane < cnt) {
+            const uint32_t j = (uint32_t)s_list[w][c + lane] & (uint32_t)(B - 1);
+            cm[lane] = s_ent[buf][0][j];
+            ct[lane] = s_ent[buf][1][j];
+            cc[lane] = s_ent[buf][2][j];
+          }
+          __builtin_amdgcn_wave_barrier();
           uint32_t mask = 0;
           for (uint32_t k = 0; k < cnt; k += 4) {
-            // past cnt the slots may be stale or uninitialised: masked into
-            // [0, B) they index s_ent, and their bits are cleared below
-            const uint32_t js = *reinterpret_cast<const uint32_t*>(&s_list[w][c + k]);
+            // slots at or past cnt hold stale entries: their bits are
+            // cleared below
             uint32_t nib = 0;
-            if constexpr (unaligned) {
-              // the half-extent is margin * S for every entry: only the map
-              // and the translation are read
-              float4 m[4];
-              float2 t[4];
 #pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                const uint32_t j = (js >> (8 * q)) & (uint32_t)(B - 1);
-                m[q] = s_ent[buf][0][j];
-                t[q] = *reinterpret_cast<const float2*>(&s_ent[buf][1][j]);
+            for (int q = 0; q < 4; ++q) {
+              float u, w2;
+              float4 t4;
+              if constexpr (unaligned) {
+                // the half-extent is margin * S for every entry
+                const float2 t2 = *reinterpret_cast<const float2*>(&ct[k + q]);
+                t4 = make_float4(t2.x, t2.y, margin * kGaussScale, 0.0f);
+              } else {
+                t4 = ct[k + q];
               }
-              const float h = margin * kGaussScale;
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                float u, w2;
-                nib |= (covers(m[q], make_float4(t[q].x, t[q].y, h, 0.0f), u, w2) ? 1u : 0u) << q;
-              }
-            } else {
-              float4 m[4], t[4];
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                const uint32_t j = (js >> (8 * q)) & (uint32_t)(B - 1);
-                m[q] = s_ent[buf][0][j];
-                t[q] = s_ent[buf][1][j];
-              }
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                float u, w2;
-                nib |= (covers(m[q], t[q], u, w2) ? 1u : 0u) << q;
-              }
+              nib |= (covers(cm[k + q], t4, u, w2) ? 1u : 0u) << q;
             }
             mask |= nib << k;
           }
@@ -603,15 +597,13 @@ __global__ __launch_bounds__(256, MODE == 3 ? 8 : 1) void k_raster(RasterLaunch 
           while (mask) {
             const uint32_t k = (uint32_t)__builtin_ctz(mask);
             mask &= mask - 1u;
-            // each lane reads slot k's batch index from the list itself (a
-            // cross-lane read would see 0 from lanes that have left the loop)
-            const uint32_t j = (uint32_t)s_list[w][c + k] & (uint32_t)(B - 1);
-            const float4 t = s_ent[buf][1][j];
+            const float4 t = ct[k];
             float u, w2;
-            (void)covers(s_ent[buf][0][j], t, u, w2);
-            shade(u, w2, t.w, s_ent[buf][2][j]);
+            (void)covers(cm[k], t, u, w2);
+            shade(u, w2, t.w, cc[k]);
             if (tr == 0.0f) mask = 0u;
           }
+          __builtin_amdgcn_wave_barrier();
         }
       } else if constexpr (!WALK) {
         // entry-major: the wave steps through its list, every lane testing
@@ -673,7 +665,12 @@ __global__ __launch_bounds__(256, MODE == 3 ? 8 : 1) void k_raster(RasterLaunch 
       alive = __ballot(inside && tr != 0.0f);
       wave_done = alive == 0ull;
     }
-    if (more) stage(buf ^ 1);
+    if (more) {
+      // mode 3's compact runs live in the other buffer until every wave has
+      // walked
+      if constexpr (LANES) __syncthreads();
+      stage(buf ^ 1);
+    }
     buf ^= 1;
     if (__syncthreads_and(wave_done)) break;
   }

@@ -285,8 +285,12 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
   const uint32_t per_view = std::max<uint32_t>(1, cus * 4 / nv);
   // radix-sort tiles: one 1024-lane, two 512-lane or four 256-lane blocks per CU, shared
   // by the views (which sort side by side, so their look-back chains overlap)
-  const uint32_t sort_tile = (uint32_t)r->sort_threads * kSortItems;
-  const uint32_t sort_per_view = std::max<uint32_t>(1, cus * 4 * kSortThreads / r->sort_threads / nv);
+  // A single view sorts in 16384-key tiles whatever the batches use: with
+  // one view on the device the look-back chains set the sort's time, and
+  // larger tiles shorten them (the autotuner picks the batches' tile).
+  const int sort_threads = nv == 1 ? kSortThreadsXL : r->sort_threads;
+  const uint32_t sort_tile = (uint32_t)sort_threads * kSortItems;
+  const uint32_t sort_per_view = std::max<uint32_t>(1, cus * 4 * kSortThreads / sort_threads / nv);
   const size_t cap = r->cap;
   const size_t sort_stride = r->sort_words * kMaxDepthPasses;
   const size_t pair_stride_status = r->pair_words * kMaxPairPasses;
@@ -377,7 +381,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a_words = sort_used;
       a.clear_a_stride = sort_stride;
     }
-    a.threads = r->sort_threads;
+    a.threads = sort_threads;
     a.grid_x = clampg(div_up(n, sort_tile), sort_per_view);
     a.views = (int)nv;
     // keys are below 2^30 (0 < clip.z < 1): the last 8-bit pass needs 6 bits
@@ -436,7 +440,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a_words = r->pair_words;
       a.clear_a_stride = pair_stride_status;
     }
-    a.threads = r->sort_threads;
+    a.threads = sort_threads;
     a.grid_x = clampg(div_up(r->pair_cap, sort_tile), sort_per_view);
     a.views = (int)nv;
     // tile keys are below max_tiles: the last pass takes only their top bits

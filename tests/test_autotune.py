@@ -27,10 +27,11 @@ def test_preference_order_and_tie(monkeypatch):
         f -= 60.0 * c.raster_longest_first + 100.0 * (c.raster_batch == 256)
         return f
     res, order = _run(monkeypatch, fps_of)
-    assert order[0] == "r10-t16384-b128-m0-o0-v16" and order[1] == "r8-t16384-b128-m0-o0-v16"
+    assert order[0] == "r10-t16384-b128-m3-o0-v16" and order[1] == "r8-t16384-b128-m3-o0-v16"
     assert res.best.sort_tile_keys == 16384 and res.best.radix_bits_per_digit == 10
     assert res.best.raster_batch == 128 and res.best.raster_longest_first == 0
-    assert len(res.table) == 6 + 4
+    assert res.best.raster_mode == 3  # the preferred walk, no slower than entry-major here
+    assert len(res.table) == 6 + 8
 
 
 def test_clear_win_replaces_the_pick(monkeypatch):

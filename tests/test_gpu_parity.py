@@ -123,7 +123,8 @@ def test_autotuner_options(bits, tile, mode, batch, longest):
     check_case(case, conics=False)
 
 
-@pytest.mark.parametrize("mode,unaligned,blend", [(1, True, sa.BlendMode.F32), (1, False, sa.BlendMode.UNORM8),
+@pytest.mark.parametrize("mode,unaligned,blend", [(0, True, sa.BlendMode.F32), (0, False, sa.BlendMode.UNORM8),
+                                                  (1, True, sa.BlendMode.F32), (1, False, sa.BlendMode.UNORM8),
                                                   (2, True, sa.BlendMode.F32), (2, False, sa.BlendMode.UNORM8),
                                                   (3, True, sa.BlendMode.F32), (3, False, sa.BlendMode.UNORM8)])
 def test_raster_modes_other_rectangles_and_blends(mode, unaligned, blend):
@@ -213,14 +214,14 @@ def test_deterministic_and_frame_size_changes():
     assert np.array_equal(a, b) and np.array_equal(ka, kb) and np.array_equal(ia, ib)
 
 
-@pytest.mark.parametrize("graphs,longest", [(True, 0), (False, 0), (True, 1)])
-def test_render_views_equals_single_frames(graphs, longest):
+@pytest.mark.parametrize("graphs,longest,mode", [(True, 0, 3), (False, 0, 3), (True, 1, 3), (True, 0, 0)])
+def test_render_views_equals_single_frames(graphs, longest, mode):
     """Batched form: uneven frame sizes in one launch sequence, more views
     than max_views (two batches), each view equal to its single render and
     to the oracle (entries bit-exact, image within tolerance)."""
     w = Workload("clustered", 300_000, seed=8)
-    case = Case(w, config(max_views=4, use_hip_graphs=graphs, raster_longest_first=longest), 640, 360,
-                orbit_poses(6)[0])
+    case = Case(w, config(max_views=4, use_hip_graphs=graphs, raster_longest_first=longest, raster_mode=mode),
+                640, 360, orbit_poses(6)[0])
     poses = orbit_poses(6)
     sizes = [(640, 360), (320, 240), (1280, 720), (97, 61), (640, 360), (500, 500)]
     frames = [sa.Frame(device(), wd, ht) for wd, ht in sizes]
