@@ -171,6 +171,56 @@ def cpu_baseline(positions: np.ndarray, poses, W: int, H: int, budget_s: float =
                              "100k Gaussians, 1280x720, 60 deg, camera at distance 5, 1 thread, median of 10"}
 
 
+def sort_stress(sa, dev, log, n: int = 32_000_000, width: int = 3840, height: int = 2160, views: int = 8) -> dict:
+    """The depth sort on the stress workload: 32M splats in 16 tight
+    clusters (sigma 0.05, dense and tie-heavy f32 depths), 8 orbit views at
+    3840x2160 in one batch.  Per-frame stage times from HIP events around
+    every stage of 4 batches, for the two sort shapes the tuner weighs
+    (10-bit digits in 16384-key tiles, 8-bit digits in 4096-key tiles);
+    Gkeys/s = visible entries / depth-sort time, GB/s = 16 B x passes x
+    visible / depth-sort time (key and value read and written per pass)."""
+    from splatter_amd.synth import Workload, orbit_poses
+    t0 = time.perf_counter()
+    shapes = [(10, 16384), (8, 4096)]
+    rs = [sa.Renderer(dev, sa.Configuration(spherical_harmonics_order=3, max_splat_count=n, max_views=views,
+                                            tile_pairs_per_splat=8, radix_bits_per_digit=b, sort_tile_keys=t))
+          for b, t in shapes]
+    scene = sa.Scene(rs[0], n)
+    for first, rec in Workload("tight", n, seed=4).chunks():
+        scene.load_records(first, rec)
+    dev.synchronize()
+    log(f"sort stress: {n} splats loaded in {time.perf_counter() - t0:.1f}s")
+    poses = orbit_poses(views, radius=9.0)
+    frames = [sa.Frame(dev, width, height) for _ in range(views)]
+    out = {"workload": f"{n // 1_000_000}M splats in 16 tight clusters (sigma 0.05), {width}x{height}, "
+                       f"{views} views per batch"}
+    for (bits, tile), r in zip(shapes, rs):
+        r.render_views(frames, [p[0] for p in poses], [p[1] for p in poses], scene)
+        r.enable_timing(True)
+        for _ in range(4):
+            r.render_views(frames, [p[0] for p in poses], [p[1] for p in poses], scene)
+        st = r.stage_times()
+        r.enable_timing(False)
+        vis = []
+        for v in range(views):
+            r.select_view(v)
+            vis.append(r.visible_count())
+        r.select_view(0)
+        V = float(np.mean(vis))
+        P = math.ceil(30 / bits)
+        sort_ms = float(st[STAGES.index("depth_sort")])
+        pair_ms = float(st[STAGES.index("pair_sort")])
+        gbs = 16.0 * P * V / (sort_ms * 1e-3) / 1e9
+        out[f"r{bits}_t{tile}"] = {"visible_per_view": int(V), "depth_sort_ms": round(sort_ms, 4),
+                                   "gkeys_per_s": round(V / (sort_ms * 1e-3) / 1e9, 2), "passes": P,
+                                   "GB/s": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4),
+                                   "pair_sort_ms": round(pair_ms, 4)}
+        log(f"sort stress r{bits} t{tile}: {out[f'r{bits}_t{tile}']}")
+    for o in frames + [scene] + rs:
+        o.close()
+    return out
+
+
 def config0_gpu_ms(sa, dev, repeats: int = 50) -> float:
     """The plain GPU path on the small workload (100k Gaussians, 1280x720,
     60 degrees, camera at distance 5): median submit-to-ready ms per frame."""
@@ -300,6 +350,8 @@ def main() -> int:
     ap.add_argument("--sort-tile", type=int, default=0, help="force keys per sort tile (skips the autotuner)")
     ap.add_argument("--no-autotune", action="store_true", help="8-bit digits, 4096-key sort tiles")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sort-stress", action="store_true",
+                    help="skip the depth sort on the 32M tie-heavy 4K 8-view stress workload")
     ap.add_argument("--in-flight", type=int, default=2,
                     help="batches in flight: the batched loop alternates this many renderers, each with its own "
                          "frames, so one batch's raster overlaps the next one's sort and projection")
@@ -524,6 +576,10 @@ def main() -> int:
                 f"{serve_high['frames_per_s']} frames/s, mean batch {serve_high['mean_batch_views']}")
         server.close()
 
+    stress = None
+    if not args.no_sort_stress:
+        stress = sort_stress(sa, dev, log)
+
     cpu = None
     if not args.no_cpu:
         c0_gpu = config0_gpu_ms(sa, dev)
@@ -567,6 +623,7 @@ def main() -> int:
         "pairs_per_view": int(K),
         "stages": per_stage,
         "raster_tiles": tiles_info,
+        "sort_stress": stress,
         "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_batch": launches,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name),

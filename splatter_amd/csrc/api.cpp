@@ -272,7 +272,8 @@ void decode_range(const SerializedSplat* in, size_t lo, size_t hi, StagingSlot& 
     for (int k = 0; k < 3; ++k)
       e[k] = expf(fminf(fmaxf(s.scale[k], average - kMaxSizeVariance), average + kMaxSizeVariance));
     st.sa[i] = make_float4(e[0], e[1], e[2], 1.0f / (1.0f + expf(-s.alpha)));
-    std::memcpy(&st.sh[48 * i], s.color, 48 * sizeof(float));
+    // each coefficient times its basis constant (the product the lookup starts with)
+    for (int k = 0; k < 48; ++k) st.sh[48 * i + k] = kShConstants[k / 3] * s.color[k];
   }
 }
 
@@ -461,10 +462,11 @@ int spl_scene_new_dev_test(spl_renderer* r, spl_scene** out) {
       st->centers[3 * i + 1] = i == 1 ? 1.0f : 0.0f;
       st->centers[3 * i + 2] = i == 2 ? 1.0f : 0.0f;
       st->sa[i] = make_float4(0.2f, 0.1f, 0.05f, 1.0f);
-      for (int k = 0; k < 48; ++k) st->sh[48 * i + k] = 0.0f;
-      st->sh[48 * i + 0] = i == 0 ? 1.0f : 0.0f;
-      st->sh[48 * i + 1] = i == 1 ? 1.0f : 0.0f;
-      st->sh[48 * i + 2] = i == 2 ? 1.0f : 0.0f;
+      // stored times the basis constants, like decoded coefficients
+      for (int k = 0; k < 48; ++k) st->sh[48 * i + k] = kShConstants[k / 3] * 0.0f;
+      st->sh[48 * i + 0] = kShConstants[0] * (i == 0 ? 1.0f : 0.0f);
+      st->sh[48 * i + 1] = kShConstants[0] * (i == 1 ? 1.0f : 0.0f);
+      st->sh[48 * i + 2] = kShConstants[0] * (i == 2 ? 1.0f : 0.0f);
     }
     rc = up.send(*st, 0, (size_t)std::min<uint64_t>(3, s->count));
   }

@@ -156,14 +156,10 @@ __device__ void covariance2(const double T[3][3], const double pos[3], const Uni
   c01 = (w0[0] * w1[0] + w0[1] * w1[1]) + w0[2] * w1[2];
 }
 
-// shaders.wgsl:248-294 (f32; same constants and evaluation order)
-__constant__ float kShc[16] = {
-    0.28209479177387814f, -0.4886025119029199f, 0.4886025119029199f, -0.4886025119029199f,
-    1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f, -1.0925484305920792f,
-    0.5462742152960396f, -0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
-    0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f, -0.5900435899266435f,
-};
-
+// shaders.wgsl:248-294 (f32; same evaluation order).  The scene holds each
+// coefficient already multiplied by its basis constant (kShConstants, at
+// load): every term of the reference starts with that product, so the f32
+// values are the same and a (splat, view) saves 48 multiplications.
 template <int L>
 __device__ __forceinline__ void sh_color(const float d[3], const float* sh, float out[3]) {
   const float x = d[0], y = d[1], z = d[2];
@@ -171,27 +167,27 @@ __device__ __forceinline__ void sh_color(const float d[3], const float* sh, floa
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     float col = 0.5f;
-    col = col + kShc[0] * sh[c];
+    col = col + sh[c];
     if (L > 0) {
-      col = col + (kShc[1] * sh[3 + c]) * y;
-      col = col + (kShc[2] * sh[6 + c]) * z;
-      col = col + (kShc[3] * sh[9 + c]) * x;
+      col = col + sh[3 + c] * y;
+      col = col + sh[6 + c] * z;
+      col = col + sh[9 + c] * x;
     }
     if (L > 1) {
-      col = col + ((kShc[4] * sh[12 + c]) * x) * y;
-      col = col + ((kShc[5] * sh[15 + c]) * y) * z;
-      col = col + ((kShc[6] * sh[18 + c]) * ((2.0f * zz - xx) - yy));
-      col = col + ((kShc[7] * sh[21 + c]) * x) * z;
-      col = col + (kShc[8] * sh[24 + c]) * (xx - yy);
+      col = col + (sh[12 + c] * x) * y;
+      col = col + (sh[15 + c] * y) * z;
+      col = col + (sh[18 + c] * ((2.0f * zz - xx) - yy));
+      col = col + (sh[21 + c] * x) * z;
+      col = col + sh[24 + c] * (xx - yy);
     }
     if (L > 2) {
-      col = col + ((kShc[9] * sh[27 + c]) * y) * (3.0f * xx - yy);
-      col = col + (((kShc[10] * sh[30 + c]) * x) * y) * z;
-      col = col + ((kShc[11] * sh[33 + c]) * y) * ((4.0f * zz - xx) - yy);
-      col = col + ((kShc[12] * sh[36 + c]) * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
-      col = col + ((kShc[13] * sh[39 + c]) * x) * ((4.0f * zz - xx) - yy);
-      col = col + ((kShc[14] * sh[42 + c]) * z) * (xx - yy);
-      col = col + ((kShc[15] * sh[45 + c]) * x) * (xx - 3.0f * yy);
+      col = col + (sh[27 + c] * y) * (3.0f * xx - yy);
+      col = col + ((sh[30 + c] * x) * y) * z;
+      col = col + (sh[33 + c] * y) * ((4.0f * zz - xx) - yy);
+      col = col + (sh[36 + c] * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+      col = col + (sh[39 + c] * x) * ((4.0f * zz - xx) - yy);
+      col = col + (sh[42 + c] * z) * (xx - yy);
+      col = col + (sh[45 + c] * x) * (xx - 3.0f * yy);
     }
     out[c] = col;
   }

@@ -57,6 +57,16 @@ struct SceneView {
   uint32_t count;
 };
 
+// shaders.wgsl:249-266: the spherical-harmonics basis constants.  A scene
+// stores coefficient k of every colour channel multiplied by constant k
+// (the product each term of the reference's lookup starts with).
+constexpr float kShConstants[16] = {
+    0.28209479177387814f, -0.4886025119029199f, 0.4886025119029199f, -0.4886025119029199f,
+    1.0925484305920792f, -1.0925484305920792f, 0.31539156525252005f, -1.0925484305920792f,
+    0.5462742152960396f, -0.5900435899266435f, 2.890611442640554f, -0.4570457994644658f,
+    0.3731763325901154f, -0.4570457994644658f, 1.445305721320277f, -0.5900435899266435f,
+};
+
 constexpr int kCullThreads = 256, kCullItems = 16;   // 4096 splats per cull / project tile
 constexpr int kSortThreads = 256, kSortItems = 16;   // 4096 keys per sort tile (the narrow form)
 constexpr int kSortThreadsWide = 512;                // 8192 keys per sort tile
