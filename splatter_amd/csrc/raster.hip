@@ -268,15 +268,20 @@ void launch_tile_order(const OrderLaunch& a, hipStream_t s) {
 // (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
 // stops blending, and the block stops when all four have.
 template <bool UNALIGNED, bool F32, int MODE, int B>
-__global__ __launch_bounds__(256, MODE == 3 ? 8 : 1) void k_raster(RasterLaunch a) {
+__global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_raster(RasterLaunch a) {
   static_assert(B == 128 || B == 256, "entries staged per batch");
   constexpr int U = B / 64;  // list slots per step of the entry-major walks
   constexpr bool WALK = MODE == 1, SUBS = MODE == 2, LANES = MODE == 3;
+  constexpr bool LANES_ = LANES;
   // per staged entry: [0] i00 i01 i10 i11, [1] tx ty half alpha, [2] r g b -
   // (one array, so an entry's three rows share one LDS address)
   __shared__ float4 s_ent[2][3][B];
   __shared__ uint2 s_box[2][B];    // pixel box lo / hi (x | y << 16)
   __shared__ __align__(16) uint8_t s_list[4][B];  // per-wave compacted entry slots, read 4 at a time
+  // mode 3: per wave, the coverage of 32 run slots as 8 rows x 8 words; byte
+  // 3 - (k >> 3) of word k & 7 of row y holds slot k's row mask (bit x =
+  // column x of the quadrant)
+  __shared__ __align__(16) uint32_t s_rows[LANES_ ? 4 : 1][LANES_ ? 64 : 4];
   __shared__ __align__(16) uint8_t s_sub[SUBS ? 4 : 1][SUBS ? 4 : 1][SUBS ? B : 4];  // mode 2: per 4x4 sub-block
 
   // Block -> (view, tile).  First the heavy tiles of every view (their
@@ -553,16 +558,54 @@ __global__ __launch_bounds__(256, MODE == 3 ? 8 : 1) void k_raster(RasterLaunch 
         // per-lane walk: for 32 list slots at a time the wave first copies
         // the slots' entries into a compact run (its 32 entries of the
         // other staging buffer, free until the next batch is staged), then
-        // every lane marks, slot after slot, the entries whose rectangle
-        // covers its pixel (entry-uniform LDS reads, one coverage bit per
-        // slot), and walks only those, in list order, each lane on its own:
+        // finds, for every pixel, the run's entries that can cover it, and
+        // every lane walks only those, in list order, each lane on its own:
         // the wave takes as many steps as its busiest pixel has covering
-        // entries, not as many as the list has.  The fragment arithmetic is
-        // the entry-major walk's, in the same order per pixel, so the pixels
-        // are the same.
+        // entries, not as many as the list has.
+        //
+        // The candidates come per row, entry-parallel: 2, 4 or 8 lanes per
+        // entry each take 4, 2 or 1 of the quadrant's rows and solve the
+        // rectangle's two slabs |a px + b py + c| < h for the row's column
+        // interval, widened by far more than the rounding of either that
+        // solution or the per-pixel test (2^-19 of the magnitudes, against
+        // about 12 roundings of 2^-24), so it holds every column whose
+        // centre the per-pixel test accepts.  A slab that hardly varies
+        // along x (|a| < h 1e-6), or with a coefficient that is not finite,
+        // is taken as all columns.  Each pixel lane
+        // then gathers its row's 32 bits from LDS, and the walk applies the
+        // exact per-pixel test of the other modes to every candidate (a
+        // candidate that fails it is blended with opacity 0, which leaves
+        // the pixel as it is), so the pixels are the same.
         float4* const cm = &s_ent[buf ^ 1][0][32 * w];
         float4* const ct = &s_ent[buf ^ 1][1][32 * w];
         float4* const cc = &s_ent[buf ^ 1][2][32 * w];
+        uint8_t* const rows8 = reinterpret_cast<uint8_t*>(&s_rows[w][0]);
+        constexpr uint32_t kRowsMin = 12;  // runs up to this long are tested per pixel
+        const float fqx0 = (float)qx0, fqy0 = (float)qy0;
+        const float span = Wf + Hf;  // bounds the pixel coordinates' magnitudes
+        // column interval of row y (local, 0..7) as [ceil(lo), floor(hi1) - 1],
+        // clamped to the quadrant, as a row mask
+        auto row_mask = [&](float lo, float hi1) -> uint32_t {
+          const uint32_t first = (uint32_t)__builtin_amdgcn_fmed3f(ceilf(lo), 0.0f, 8.0f);
+          const uint32_t end = (uint32_t)__builtin_amdgcn_fmed3f(floorf(hi1), 0.0f, 8.0f);
+          return ((1u << end) - 1u) & ~((1u << first) - 1u);
+        };
+        // slab |a px + b py + c| < h in local columns: lo(y) = P y + lo0,
+        // hi(y) + 1 = P y + hi0
+        auto slab = [&](float a_, float b_, float c_, float h_, float& P, float& lo0, float& hi0) {
+          const float ia = __builtin_amdgcn_rcpf(a_);
+          const float R = h_ * fabsf(ia);
+          const float Pv = -b_ * ia;
+          const float Qg = -c_ * ia;
+          const float Q = fmaf(Pv, fqy0, Qg) - fqx0;
+          const float eps = 0x1p-19f * (((fabsf(Pv) * span + fabsf(Qg)) + R) + span) + 0x1p-19f;
+          // (R = 0 for an infinite a, whose products with a zero
+          // coordinate are NaN in the per-pixel test)
+          const bool ok = R > 0.0f && R < 1e6f && fabsf(Pv) < 1e6f && fabsf(Qg) < 1e9f;
+          P = ok ? Pv : 0.0f;
+          lo0 = ok ? (Q - R) - eps : -1e9f;
+          hi0 = ok ? ((Q + R) + eps) + 1.0f : 1e9f;
+        };
         for (uint32_t c = 0; c < n; c += 32) {
           const uint32_t cnt = min(32u, n - c);
           if (lane < cnt) {
@@ -573,34 +616,86 @@ __global__ __launch_bounds__(256, MODE == 3 ? 8 : 1) void k_raster(RasterLaunch 
           }
           __builtin_amdgcn_wave_barrier();
           uint32_t mask = 0;
-          for (uint32_t k = 0; k < cnt; k += 4) {
-            // slots at or past cnt hold stale entries: their bits are
-            // cleared below
-            uint32_t nib = 0;
+          if (cnt <= kRowsMin) {
+            // a short run: every lane tests its pixel against each slot
+            // (exact), slot k at bit K - 1 - k for K = cnt rounded up to 4
+            for (uint32_t k = 0; k < cnt; k += 4) {
+              uint32_t nib = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              float u, w2;
-              float4 t4;
-              if constexpr (unaligned) {
-                // the half-extent is margin * S for every entry
-                const float2 t2 = *reinterpret_cast<const float2*>(&ct[k + q]);
-                t4 = make_float4(t2.x, t2.y, margin * kGaussScale, 0.0f);
-              } else {
-                t4 = ct[k + q];
+              for (int q = 0; q < 4; ++q) {
+                float u, w2;
+                float4 t4;
+                if constexpr (unaligned) {
+                  // the half-extent is margin * S for every entry
+                  const float2 t2 = *reinterpret_cast<const float2*>(&ct[k + q]);
+                  t4 = make_float4(t2.x, t2.y, margin * kGaussScale, 0.0f);
+                } else {
+                  t4 = ct[k + q];
+                }
+                nib = (nib << 1) | (covers(cm[k + q], t4, u, w2) ? 1u : 0u);
               }
-              nib |= (covers(cm[k + q], t4, u, w2) ? 1u : 0u) << q;
+              mask = (mask << 4) | nib;
             }
-            mask |= nib << k;
+            mask <<= 32u - ((cnt + 3u) & ~3u);
+          } else {
+            {
+              const uint32_t lpe = cnt <= 8u ? 3u : (cnt <= 16u ? 2u : 1u);  // log2 lanes per entry
+              const uint32_t k = lane >> lpe;
+              if (k < cnt) {
+                const uint32_t rpl = 8u >> lpe;  // rows per lane
+                const uint32_t y0 = (lane & ((1u << lpe) - 1u)) * rpl;
+                const float4 m = cm[k];
+                const float4 t = ct[k];
+                uint8_t* const dst = rows8 + (k & 7u) * 4u + (3u - (k >> 3));
+                if constexpr (unaligned) {
+                  const float h = margin * kGaussScale;
+                  float P1, l1, e1, P2, l2, e2;
+                  slab(m.x, m.y, t.x, h, P1, l1, e1);
+                  slab(m.z, m.w, t.y, h, P2, l2, e2);
+                  for (uint32_t i = 0; i < rpl; ++i) {
+                    const float y = (float)(y0 + i);
+                    const float lo = fmaxf(fmaf(P1, y, l1), fmaf(P2, y, l2));
+                    const float hi1 = fminf(fmaf(P1, y, e1), fmaf(P2, y, e2));
+                    dst[(y0 + i) * 32u] = (uint8_t)row_mask(lo, hi1);
+                  }
+                } else {
+                  // dx = sx px + t.x bounds the columns, dy = sy py + t.y the rows
+                  float P, lx, ex, Py, ly, ey;
+                  slab(sx, 0.0f, t.x, t.z, P, lx, ex);
+                  slab(sy, 0.0f, t.y, t.z, Py, ly, ey);
+                  // rows: the same interval, in rows (py in place of px)
+                  ly += fqx0 - fqy0;
+                  ey += fqx0 - fqy0;
+                  const uint32_t cols = row_mask(lx, ex);
+                  const uint32_t rmask = row_mask(ly, ey);
+                  for (uint32_t i = 0; i < rpl; ++i)
+                    dst[(y0 + i) * 32u] = (uint8_t)(((rmask >> (y0 + i)) & 1u) ? cols : 0u);
+                }
+              }
+            }
+            __builtin_amdgcn_wave_barrier();
+            // this pixel's candidates: slot k at bit 31 - k
+            const uint32_t xl = lane & 7u;
+            const uint4 ra = *reinterpret_cast<const uint4*>(&s_rows[w][(lane >> 3) * 8u]);
+            const uint4 rb = *reinterpret_cast<const uint4*>(&s_rows[w][(lane >> 3) * 8u + 4u]);
+            mask = (mask << 1) | ((ra.x >> xl) & 0x01010101u);
+            mask = (mask << 1) | ((ra.y >> xl) & 0x01010101u);
+            mask = (mask << 1) | ((ra.z >> xl) & 0x01010101u);
+            mask = (mask << 1) | ((ra.w >> xl) & 0x01010101u);
+            mask = (mask << 1) | ((rb.x >> xl) & 0x01010101u);
+            mask = (mask << 1) | ((rb.y >> xl) & 0x01010101u);
+            mask = (mask << 1) | ((rb.z >> xl) & 0x01010101u);
+            mask = (mask << 1) | ((rb.w >> xl) & 0x01010101u);
           }
-          if (cnt < 32) mask &= (1u << cnt) - 1u;
+          mask &= ~0u << (32u - cnt);  // slots at or past cnt hold stale rows
           if (tr == 0.0f) mask = 0u;
           while (mask) {
-            const uint32_t k = (uint32_t)__builtin_ctz(mask);
-            mask &= mask - 1u;
+            const uint32_t k = (uint32_t)__builtin_clz(mask);
+            mask &= ~(0x80000000u >> k);
             const float4 t = ct[k];
             float u, w2;
-            (void)covers(cm[k], t, u, w2);
-            shade(u, w2, t.w, cc[k]);
+            const bool in = covers(cm[k], t, u, w2);
+            shade(u, w2, in ? t.w : 0.0f, cc[k]);
             if (tr == 0.0f) mask = 0u;
           }
           __builtin_amdgcn_wave_barrier();
