@@ -500,7 +500,9 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
           return fmaxf(fabsf(dx), fabsf(dy)) < h;
         }
       };
-      auto shade = [&](float u, float w2, float opacity, const float4& col) {
+      // (in: the coverage test's verdict, for walks over candidates that
+      // did not pass it yet; an uncovered fragment blends nothing)
+      auto shade = [&](float u, float w2, float opacity, const float4& col, bool in) {
         float alpha = opacity * spl_exp2(-fmaf(u, u, w2 * w2));
         if constexpr (!f32) {
           // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
@@ -508,7 +510,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
           // threshold alpha becomes 0, which leaves C and T as they are
           // (they hold integers; clamp01 of a NaN product is 0), so no
           // branch is needed.
-          alpha = alpha >= 1.0f / 255.0f ? alpha : 0.0f;
+          alpha = in && alpha >= 1.0f / 255.0f ? alpha : 0.0f;
           const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
           // the source alpha needs no clamp: the opacity is a sigmoid (or 1
           // in the dev-test scene) and spl_exp2 of a non-positive argument is
@@ -519,7 +521,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
           c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
           c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
           tr = rintf(fmaf(-sa, tr, tr));
-        } else if (alpha >= 1.0f / 255.0f) {
+        } else if (in && alpha >= 1.0f / 255.0f) {
           const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
           const float da = tr;
           c0 = fmaf(sr, da, c0);
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
 #pragma unroll
           for (int k = 0; k < U; ++k) {
             float u, w2;
-            if (i + k < mine && covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k]);
+            if (i + k < mine && covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k], true);
           }
         }
       } else if constexpr (LANES) {
@@ -574,8 +576,8 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
         // is taken as all columns.  Each pixel lane
         // then gathers its row's 32 bits from LDS, and the walk applies the
         // exact per-pixel test of the other modes to every candidate (a
-        // candidate that fails it is blended with opacity 0, which leaves
-        // the pixel as it is), so the pixels are the same.
+        // candidate that fails it blends with alpha 0, which leaves the
+        // pixel as it is), so the pixels are the same.
         float4* const cm = &s_ent[buf ^ 1][0][32 * w];
         float4* const ct = &s_ent[buf ^ 1][1][32 * w];
         float4* const cc = &s_ent[buf ^ 1][2][32 * w];
@@ -695,7 +697,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
             const float4 t = ct[k];
             float u, w2;
             const bool in = covers(cm[k], t, u, w2);
-            shade(u, w2, in ? t.w : 0.0f, cc[k]);
+            shade(u, w2, t.w, cc[k], in);
             if (tr == 0.0f) mask = 0u;
           }
           __builtin_amdgcn_wave_barrier();
@@ -722,7 +724,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
           for (int k = 0; k < U; ++k) {
             if (i + k >= n) break;
             float u, w2;
-            if (covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k]);
+            if (covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k], true);
           }
         }
       } else {
@@ -752,7 +754,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
             const float4 t = s_ent[buf][1][j];
             float u, w2;
             (void)covers(s_ent[buf][0][j], t, u, w2);
-            shade(u, w2, t.w, s_ent[buf][2][j]);
+            shade(u, w2, t.w, s_ent[buf][2][j], true);
             if (tr == 0.0f) lo = hi = 0u;
           }
         }
