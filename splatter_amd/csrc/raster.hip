@@ -285,19 +285,29 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
   __shared__ __align__(16) uint8_t s_sub[SUBS ? 4 : 1][SUBS ? 4 : 1][SUBS ? B : 4];  // mode 2: per 4x4 sub-block
 
   // Block -> (view, tile).  First the heavy tiles of every view (their
-  // count rounded up to 8 blocks), then each view's other tiles in a slot
-  // range of slots_per_view blocks, banded per XCD: blocks are dealt
-  // round-robin to the 8 XCDs, and each XCD takes a contiguous run of rows
-  // (neighbouring tiles share records in its L2).
+  // count rounded up to 8 blocks; with split_heavy, mode 3 only, four blocks
+  // per heavy tile, one per 8x8 quadrant, each wave two of its rows), then
+  // each view's other tiles in a slot range of slots_per_view blocks,
+  // banded per XCD: blocks are dealt round-robin to the 8 XCDs, and each
+  // XCD takes a contiguous run of rows (neighbouring tiles share records in
+  // its L2).  A heavy tile's pixels wait on their own long chains of
+  // blends; with one view on the device the other blocks are soon done, and
+  // sixteen waves on a tile instead of four shorten its chains' wave-wide
+  // steps (each step waits for the busiest of 16 pixels, not 64).
   const uint32_t nv = (uint32_t)a.views;
-  uint32_t v = 0, pos = 0;
+  uint32_t v = 0, pos = 0, quad = 4;  // quad < 4: the quadrant of a split heavy tile
   {
     uint32_t heavy_total = 0;
     if (a.longest_first)
       for (uint32_t i = 0; i < nv; ++i) heavy_total += a.counters[(size_t)i * kCounterWords + kHeavyTiles];
-    const uint32_t heavy_blocks = (heavy_total + 7u) & ~7u;
+    const uint32_t split = LANES && a.split_heavy ? 4u : 1u;
+    const uint32_t heavy_blocks = ((heavy_total + 7u) & ~7u) * split;
     uint32_t b = blockIdx.x;
     if (b < heavy_blocks) {
+      if (split == 4u) {
+        quad = b & 3u;
+        b >>= 2;
+      }
       if (b >= heavy_total) return;
       for (;; ++v) {
         const uint32_t h = a.counters[(size_t)v * kCounterWords + kHeavyTiles];
@@ -318,22 +328,24 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
     }
   }
   const Uniforms& u = a.uniforms[v];
-  uint2* ranges = a.ranges + v * a.ranges_stride;
+  const uint2* ranges = a.ranges + v * a.ranges_stride;
   const uint32_t* pair_vals = a.pair_vals + v * a.pair_stride;
   const float4* records = a.records + v * a.rec_stride;
   const uint32_t tile = a.longest_first ? a.order[v * a.order_stride + pos] : pos;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  const uint32_t qx0 = (tile % u.tiles_x) * kTileSize + (w & 1u) * 8u;
-  const uint32_t qy0 = (tile / u.tiles_x) * kTileSize + (w >> 1) * 8u;
+  // this wave's 8x8 region: quadrant w, or for a split tile rows 2w and
+  // 2w + 1 of quadrant quad (lanes 0-15; the others hold no pixel)
+  const uint32_t qd = quad < 4u ? quad : w;
+  const uint32_t qx0 = (tile % u.tiles_x) * kTileSize + (qd & 1u) * 8u;
+  const uint32_t qy0 = (tile / u.tiles_x) * kTileSize + (qd >> 1) * 8u + (quad < 4u ? 2u * w : 0u);
   // mode 2: lane group g = lane / 16 owns the 4x4 sub-block (g & 1, g >> 1)
   const uint32_t g = lane >> 4;
   const uint32_t px = SUBS ? qx0 + (g & 1u) * 4u + (lane & 3u) : qx0 + (lane & 7u);
   const uint32_t py = SUBS ? qy0 + (g >> 1) * 4u + ((lane >> 2) & 3u) : qy0 + (lane >> 3);
   const uint32_t W = u.image_size[0], H = u.image_size[1];
-  const bool inside = px < W && py < H;
+  const bool inside = px < W && py < H && (quad == 4u || lane < 16u);
+  // (the ranges are cleared for the next frame by its pair sort)
   const uint2 range = ranges[tile];
-  __syncthreads();
-  if (tid == 0) ranges[tile] = make_uint2(0u, 0u);
 
   // Pixel centres in the view plane are q = (sx px + ox, sy py + oy)
   // (shaders.wgsl:561 inverted).  Each staged entry folds this map into its
@@ -690,7 +702,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
             mask = (mask << 1) | ((rb.w >> xl) & 0x01010101u);
           }
           mask &= ~0u << (32u - cnt);  // slots at or past cnt hold stale rows
-          if (tr == 0.0f) mask = 0u;
+          if (tr == 0.0f || !inside) mask = 0u;
           while (mask) {
             const uint32_t k = (uint32_t)__builtin_clz(mask);
             mask &= ~(0x80000000u >> k);

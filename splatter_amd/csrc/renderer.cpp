@@ -435,6 +435,11 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
       a.clear_a = reinterpret_cast<uint32_t*>(r->bin_status);
       a.clear_a_words = (size_t)div_up(n, kBinThreads * kBinItems) * 2;
       a.clear_a_stride = r->bin_words * 2;
+      // the tile ranges the previous frame's raster read: tiles without
+      // pairs in this frame must read as empty
+      a.clear_b = reinterpret_cast<uint32_t*>(r->ranges);
+      a.clear_b_words = (size_t)b.max_tiles * 2;
+      a.clear_b_stride = (size_t)r->ranges_cap * 2;
     } else {
       a.clear_a = r->pair_status + (size_t)(p - 1) * r->pair_words;
       a.clear_a_words = r->pair_words;
@@ -505,7 +510,9 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.order = r->tile_order; a.order_stride = r->ranges_cap;
     a.slots_per_view = slots;
     a.heavy_cap = heavy_cap;
-    a.grid_x = (int)(nv * slots + (longest_first ? ((nv * heavy_cap + 7) / 8) * 8 : 0));
+    // one view in mode 3: each heavy tile's four quadrants go to four blocks
+    a.split_heavy = single && cfg.raster_mode == 3 ? 1 : 0;
+    a.grid_x = (int)(nv * slots + (longest_first ? ((nv * heavy_cap + 7) / 8) * 8 * (a.split_heavy ? 4 : 1) : 0));
     a.views = (int)nv;
     launch_raster(a, q);
   }

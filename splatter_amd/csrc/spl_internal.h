@@ -189,6 +189,7 @@ struct RasterLaunch {
                              // 3 per-lane walk over coverage masks
   int batch_entries;         // entries staged in LDS per batch: 128 or 256
   int longest_first;         // take the tiles in the order k_tile_order wrote
+  int split_heavy;           // mode 3, longest first: a heavy tile's quadrants on four blocks, two rows a wave
   int bgra;
   uint2* tile_work;          // optional, view 0: per tile (list length, entries consumed)
   int grid_x, views;
