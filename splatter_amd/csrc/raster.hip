@@ -194,11 +194,13 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
 }
 
 // Raster order of a view's tiles: the tiles with the longest pair lists
-// ("heavy": at least 8x the mean and 2048 pairs, at most heavy_cap of them,
-// whole log2 classes only) first, longest class first, then every other
-// tile in row-major order.  A heavy tile blends thousands of entries per
-// pixel, one after another; started last, it would outlast the rest of the
-// frame.
+// ("heavy": at least heavy_mult x the mean and heavy_min pairs, at most
+// heavy_cap of them, whole log2 classes only) first, longest class first,
+// then every other tile in row-major order.  A heavy tile blends thousands
+// of entries per pixel, one after another; started last, it would outlast
+// the rest of the frame.  The heavy tiles in classes of at least
+// split_mult x the mean (and 2048 pairs) are counted apart: the raster may
+// give each of them four blocks.
 __global__ __launch_bounds__(1024) void k_tile_order(OrderLaunch a) {
   constexpr int THREADS = 1024;
   __shared__ uint32_t s_cnt[32];
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(1024) void k_tile_order(OrderLaunch a) {
   uint32_t k = counters[kPairs];
   if (k > a.pair_cap) k = a.pair_cap;
   const uint32_t mean = T ? k / T : 0u;
-  const uint32_t thr = max(8u * mean, 2048u);
+  const uint32_t thr = max(a.heavy_mult * mean, a.heavy_min);
   if (tid < 32) s_cnt[tid] = 0u;
   __syncthreads();
   const uint32_t chunk = (T + THREADS - 1) / THREADS;
@@ -239,6 +241,12 @@ __global__ __launch_bounds__(1024) void k_tile_order(OrderLaunch a) {
     }
     s_cut = cut;
     counters[kHeavyTiles] = h;
+    uint32_t split = 0;
+    if (a.split_mult) {
+      const uint32_t c_split = 31u - __clz(max(a.split_mult * mean, 2048u));
+      for (uint32_t c = max(c_split, cut); c < 32u; ++c) split += s_cnt[c];
+    }
+    counters[kSplitTiles] = split;
   }
   __syncthreads();
   const uint32_t cut = s_cut;
@@ -297,18 +305,23 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
   const uint32_t nv = (uint32_t)a.views;
   uint32_t v = 0, pos = 0, quad = 4;  // quad < 4: the quadrant of a split heavy tile
   {
-    uint32_t heavy_total = 0;
+    uint32_t heavy_total = 0, split_total = 0;
     if (a.longest_first)
       for (uint32_t i = 0; i < nv; ++i) heavy_total += a.counters[(size_t)i * kCounterWords + kHeavyTiles];
-    const uint32_t split = LANES && a.split_heavy ? 4u : 1u;
-    const uint32_t heavy_blocks = ((heavy_total + 7u) & ~7u) * split;
+    // split tiles: the first of view 0's heavy tiles (splitting is for one view)
+    if (LANES && a.split_heavy && a.longest_first) split_total = a.counters[kSplitTiles];
+    const uint32_t split_blocks = ((split_total + 7u) & ~7u) * 4u;
+    const uint32_t heavy_blocks = split_blocks + ((heavy_total - split_total + 7u) & ~7u);
     uint32_t b = blockIdx.x;
     if (b < heavy_blocks) {
-      if (split == 4u) {
+      if (b < split_blocks) {
         quad = b & 3u;
         b >>= 2;
+        if (b >= split_total) return;
+      } else {
+        b = split_total + (b - split_blocks);
+        if (b >= heavy_total) return;
       }
-      if (b >= heavy_total) return;
       for (;; ++v) {
         const uint32_t h = a.counters[(size_t)v * kCounterWords + kHeavyTiles];
         if (b < h) break;

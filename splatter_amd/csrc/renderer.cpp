@@ -23,6 +23,7 @@ using namespace spl;
 namespace {
 
 constexpr int kStageSlots = 4;  // pinned uniform uploads in flight
+constexpr uint32_t kSingleHeavyMult = 2;
 constexpr int kStageCount = 8;  // cull, histograms, sort, project, bin, pair sort, ranges, raster
 
 struct UploadSlot {
@@ -488,6 +489,11 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.pair_cap = (uint32_t)r->pair_cap;
     a.order = r->tile_order; a.order_stride = r->ranges_cap;
     a.heavy_cap = heavy_cap;
+    // one view: tiles from twice the mean list go first (its time is set by
+    // its longest tiles), those from 8x the mean on four blocks each
+    a.heavy_mult = single ? kSingleHeavyMult : 8u;
+    a.heavy_min = single ? 512u : 2048u;
+    a.split_mult = single && cfg.raster_mode == 3 ? 8u : 0u;
     a.views = (int)nv;
     launch_tile_order(a, q);
   }
@@ -512,7 +518,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.heavy_cap = heavy_cap;
     // one view in mode 3: each heavy tile's four quadrants go to four blocks
     a.split_heavy = single && cfg.raster_mode == 3 ? 1 : 0;
-    a.grid_x = (int)(nv * slots + (longest_first ? ((nv * heavy_cap + 7) / 8) * 8 * (a.split_heavy ? 4 : 1) : 0));
+    a.grid_x = (int)(nv * slots + (longest_first ? ((nv * heavy_cap + 7) / 8) * 8 * (a.split_heavy ? 5 : 1) : 0));
     a.views = (int)nv;
     launch_raster(a, q);
   }

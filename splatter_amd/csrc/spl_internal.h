@@ -45,6 +45,7 @@ enum CounterSlot : int {
   kPairOverflow = 34,
   kScatterError = 35,
   kHeavyTiles = 36, // tiles the raster takes first (longest lists)
+  kSplitTiles = 37, // the first of those, that the raster splits over four blocks
   kCounterWords = 64
 };
 
@@ -170,6 +171,8 @@ struct OrderLaunch {
   uint32_t pair_cap;
   uint32_t* order; size_t order_stride;      // per view: heavy tiles, longest first, then the rest in order
   uint32_t heavy_cap;                        // at most this many heavy tiles per view
+  uint32_t heavy_mult, heavy_min;            // heavy: at least heavy_mult x the mean and heavy_min pairs
+  uint32_t split_mult;                       // split: at least split_mult x the mean and 2048 pairs (0: none)
   int views;
 };
 
