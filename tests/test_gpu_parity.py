@@ -83,6 +83,31 @@ def test_gaussian_1m_1080p_random_pose():
     check_case(case)
 
 
+@pytest.mark.parametrize("unaligned,blend", [(True, sa.BlendMode.UNORM8), (False, sa.BlendMode.UNORM8),
+                                             (True, sa.BlendMode.F32), (False, sa.BlendMode.F32)])
+def test_single_view_split_heavy_tiles(unaligned, blend):
+    """One view in mode 3 gives each tile with a list of 8x the mean (and
+    2048 pairs) four blocks, one per quadrant: the frame still equals the
+    oracle's, and equals the same view rendered in a batch (no split)."""
+    say(f"split heavy tiles, unaligned {unaligned}, blend {blend}")
+    case = Case(Workload("tight", 200_000, seed=12),
+                config(use_unaligned_rectangles=unaligned, blend_mode=blend, raster_mode=3, max_views=2),
+                800, 600, orbit_poses(8, radius=8)[5])
+    case.renderer.enable_tile_stats(True)
+    img = check_case(case, conics=False)
+    lens = case.renderer.tile_stats()[:, 0].astype(np.int64)
+    heavy = np.count_nonzero(lens >= max(8 * (lens.sum() // len(lens)), 2048))
+    say(f"  tiles {len(lens)}, mean list {lens.mean():.0f}, max {lens.max()}, split tiles {heavy}")
+    assert heavy > 0
+    other = sa.Frame(case.frame.device, case.width, case.height)
+    case.renderer.render_views([case.frame, other], [case.cam, orbit_poses(8, radius=8)[1][0]],
+                               [case.view, orbit_poses(8, radius=8)[1][1]], case.scene)
+    batch = case.frame.read()
+    if case.cfg.frame_format == sa.FrameFormat.BGRA8_UNORM:
+        batch = batch[..., [2, 1, 0, 3]]
+    assert np.array_equal(batch, img)
+
+
 def test_tie_heavy_tight_clusters():
     """Dense clusters: many equal f32 depths -> ties broken by splat index."""
     case = Case(Workload("tight", 300_000, seed=2), config(), 1280, 720, orbit_poses(8, radius=8)[3])
