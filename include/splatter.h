@@ -97,9 +97,11 @@ typedef struct spl_configuration {
                                              1: each lane walks the entries covering its pixel;
                                              2: four lane groups step through the lists of their 4x4
                                              sub-blocks side by side;
-                                             3: each lane marks the entries covering its pixel 64 list
-                                             slots at a time, then walks them on its own (all the same
-                                             image) */
+                                             3 (default): for 32 list slots at a time each lane finds
+                                             the entries that can cover its pixel (per row, from the
+                                             rectangles' column intervals), then walks them on its own;
+                                             one view also splits its heaviest tiles over four blocks
+                                             (all the same image) */
     uint32_t raster_batch;                /* entries a raster block stages per batch: 128 or 256
                                              (batches of two or more views; one view uses 256) */
     uint32_t raster_longest_first;        /* 1: tiles with the longest lists start first (same image;

@@ -8,13 +8,15 @@
 //          the digit histograms of the tile ids for the stable pair sort.
 //  ranges  [start, end) of each tile's run in the tile-sorted pair list.
 //  raster  one 256-lane block per tile, one lane per pixel, one wave per 8x8
-//          quadrant: records staged in LDS 128 or 256 at a time, each wave
-//          keeping the ones that can reach its still-transparent pixels,
-//          blended front to back in draw order with the reference's blend
-//          state (colour = src*DstAlpha + dst, alpha = dst*(1-srcA), cleared
-//          to (0,0,0,1)), the unorm8 target emulated by clamping the source
-//          and re-quantising after every blend; a wave stops once its pixels'
-//          transmittance is 0, a tile once all four have.
+//          quadrant (one view: the heaviest tiles on four blocks, one per
+//          quadrant, two rows a wave): records staged in LDS 128 or 256 at a
+//          time, each wave keeping the ones that can reach its
+//          still-transparent pixels, blended front to back in draw order with
+//          the reference's blend state (colour = src*DstAlpha + dst, alpha =
+//          dst*(1-srcA), cleared to (0,0,0,1)), the unorm8 target emulated by
+//          clamping the source and re-quantising after every blend; a wave
+//          stops once its pixels' transmittance is 0, a tile once all four
+//          have.
 #include "spl_device.h"
 #include "spl_internal.h"
 
