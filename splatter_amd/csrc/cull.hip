@@ -21,6 +21,10 @@
 
 namespace spl {
 
+struct __attribute__((aligned(4))) Float3 {
+  float x, y, z;
+};
+
 template <int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_cull_views(CullLaunch a) {
   constexpr int WAVES = THREADS / kWave;
@@ -73,16 +77,19 @@ __global__ __launch_bounds__(THREADS) void k_cull_views(CullLaunch a) {
 
     float x[ITEMS], y[ITEMS], z[ITEMS];
     uint32_t vm[ITEMS];
+    // All sixteen centres are requested before any is used: past the end a
+    // lane reads centre 0 (n > 0 here) and drops it, so that no load sits in
+    // a branch of its own, which would make each wait for the one before.
+    // One 12-byte load per centre (a single dwordx3 request per lane).
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
       const uint32_t idx = t * TILE + j * THREADS + tid;
+      const bool in = idx < n;
+      const Float3 c = *reinterpret_cast<const Float3*>(a.centers + 3 * (size_t)(in ? idx : 0u));
       vm[j] = 0u;
-      x[j] = y[j] = z[j] = 0.0f;
-      if (idx < n) {
-        x[j] = a.centers[3 * (size_t)idx + 0];
-        y[j] = a.centers[3 * (size_t)idx + 1];
-        z[j] = a.centers[3 * (size_t)idx + 2];
-      }
+      x[j] = in ? c.x : 0.0f;
+      y[j] = in ? c.y : 0.0f;
+      z[j] = in ? c.z : 0.0f;
     }
     // A: visibility per (view, item), and per view the (item, wave) counts
     for (uint32_t v = 0; v < nv; ++v) {

@@ -94,19 +94,31 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
     __syncthreads();
     const uint32_t t = s_tile;
     if (t >= num_tiles) break;
-    uint32_t cnt[ITEMS], sum = 0;
+    // The items' record indices, then their boxes, are all requested before
+    // any is used (past the end a lane reads item 0 and drops it), so that
+    // no load waits for the one before.
+    uint32_t cnt[ITEMS], sum = 0, rec[ITEMS];
+    uint2 box[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t r = t * TILE + tid * ITEMS + k;
+      rec[k] = order ? order[r < n ? r : 0u] : r;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const uint32_t r = t * TILE + tid * ITEMS + k;
+      if (r < n && rec[k] >= n) counters[kScatterError] = 1u;  // only after an upstream error
+      if (r >= n || rec[k] >= n) rec[k] = 0u;  // never read past the boxes
+      box[k] = boxes[rec[k]];
+    }
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
       const uint32_t slot = tid * ITEMS + k;
       const uint32_t r = t * TILE + slot;
       cnt[k] = 0;
       if (r < n) {
-        uint32_t c = order ? order[r] : r;
-        if (c >= n) {  // only after an upstream error; never read past the boxes
-          counters[kScatterError] = 1u;
-          c = 0;
-        }
-        const uint2 q = boxes[c];
+        const uint32_t c = rec[k];
+        const uint2 q = box[k];
         const uint32_t lo = q.x, hi = q.y;
         const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;
         const uint32_t x1 = (hi & 0xFFFFu) / kTileSize, y1 = (hi >> 16) / kTileSize;
