@@ -172,10 +172,23 @@ __global__ __launch_bounds__(256) void k_digit_histograms(HistLaunch a) {
   for (int i = threadIdx.x; i < words * REP; i += blockDim.x) s_hist[i] = 0u;
   __syncthreads();
   const uint32_t rep = threadIdx.x % REP;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const uint32_t k = keys[i];
-    for (int p = 0; p < a.passes; ++p)
-      atomicAdd(&s_hist[(p * RADIX + ((k >> (p * BITS)) & (RADIX - 1))) * REP + rep], 1u);
+  // eight keys per lane are requested before any is counted (past the end
+  // a lane reads key 0 and drops it), so the loads do not wait in turn
+  constexpr uint32_t U = 8;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += U * stride) {
+    uint32_t k[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t i = i0 + u * stride;
+      k[u] = keys[i < n ? i : 0u];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      if (i0 + u * stride >= n) break;
+      for (int p = 0; p < a.passes; ++p)
+        atomicAdd(&s_hist[(p * RADIX + ((k[u] >> (p * BITS)) & (RADIX - 1))) * REP + rep], 1u);
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < words; i += blockDim.x) {

@@ -260,6 +260,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   __shared__ float4 s_stage[THREADS][3];
   __shared__ uint2 s_dst[THREADS];      // view, entry
   __shared__ uint32_t s_wcnt[16][WAVES];  // pairs per (view, wave), then their view-major offsets
+  // the listed views' uniforms: the FP64 part reads them in LDS instead of
+  // waiting on global loads in the middle of its chains
+  __shared__ Uniforms s_u[16];
   const uint32_t nv = a.views;
   for (uint32_t v = 0; v < nv; ++v) {
     if (a.clear_a) clear_words(a.clear_a + v * a.clear_a_stride, a.clear_a_words);
@@ -312,7 +315,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         const uint32_t v = v0 + h;
         const uint32_t i = t * TILE + (it >> 4);
         const uint32_t c = s_entry[k];
-        const Uniforms& u = a.uniforms[v];
+        const Uniforms& u = s_u[h];
 
         const float cx = a.scene.centers[3 * (size_t)i + 0];
         const float cy = a.scene.centers[3 * (size_t)i + 1];
@@ -445,6 +448,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 
     for (uint32_t v0 = 0; v0 < nv; v0 += 16) {
       const uint32_t nh = min(16u, nv - v0);
+      {
+        constexpr uint32_t UW = sizeof(Uniforms) / 4;
+        static_assert(sizeof(Uniforms) % 4 == 0, "uniforms copy as words");
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.uniforms + v0);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_u);
+        for (uint32_t i = tid; i < nh * UW; i += THREADS) dst[i] = src[i];
+      }
+      __syncthreads();
       uint32_t listed = 0;  // pairs in the list (uniform across the block)
       for (int j = 0; j < ITEMS; ++j) {
         const uint32_t base = j * THREADS;

@@ -187,8 +187,11 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
   if (n > a.pair_cap) n = a.pair_cap;
   const uint32_t step = gridDim.x * blockDim.x * 4;
   for (uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4; i0 < n; i0 += step) {
+    // the three loads go out together: out of range they read key 0 (or
+    // the run's first key) and the value is replaced after
     uint32_t k[6];
-    k[0] = i0 > 0 ? keys[i0 - 1] : 0xFFFFFFFFu;
+    const uint32_t prev = keys[i0 > 0 ? i0 - 1 : 0u];
+    const uint32_t next = keys[i0 + 4 < n ? i0 + 4 : 0u];
     if (i0 + 4 <= n) {
       const uint4 q = *reinterpret_cast<const uint4*>(keys + i0);
       k[1] = q.x; k[2] = q.y; k[3] = q.z; k[4] = q.w;
@@ -196,7 +199,8 @@ __global__ __launch_bounds__(256) void k_tile_ranges(RangesLaunch a) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) k[1 + j] = i0 + j < n ? keys[i0 + j] : 0xFFFFFFFFu;
     }
-    k[5] = i0 + 4 < n ? keys[i0 + 4] : 0xFFFFFFFFu;
+    k[0] = i0 > 0 ? prev : 0xFFFFFFFFu;
+    k[5] = i0 + 4 < n ? next : 0xFFFFFFFFu;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t i = i0 + j, t = k[1 + j];
