@@ -422,14 +422,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   };
 
   for (uint32_t t = blockIdx.x; t < num_tiles; t += gridDim.x) {
-    // per view: exclusive offsets of the (item, wave) groups inside the tile
-    for (uint32_t v = 0; v < nv; ++v) {
+    // per view: exclusive offsets of the (item, wave) groups inside the tile.
+    // The sixteen masks are requested before any is used (past the end a
+    // lane reads splat 0's and drops it), so the loads do not wait in turn.
+    {
+      uint32_t vm[ITEMS];
 #pragma unroll
       for (int j = 0; j < ITEMS; ++j) {
         const uint32_t idx = t * TILE + j * THREADS + tid;
-        const uint32_t m = idx < n ? a.vis[idx] : 0u;
-        const uint64_t b = __ballot((m >> v) & 1u);
-        if (lane == 0) s_off[v][j * WAVES + w] = (uint32_t)__popcll(b);
+        vm[j] = a.vis[idx < n ? idx : 0u];
+      }
+      // (keeps the scheduler from interleaving each load with its use)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j)
+        if (t * TILE + j * THREADS + tid >= n) vm[j] = 0u;
+      for (uint32_t v = 0; v < nv; ++v) {
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+          const uint64_t b = __ballot((vm[j] >> v) & 1u);
+          if (lane == 0) s_off[v][j * WAVES + w] = (uint32_t)__popcll(b);
+        }
       }
     }
     __syncthreads();
@@ -457,9 +470,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
       __syncthreads();
       uint32_t listed = 0;  // pairs in the list (uniform across the block)
+      // each group's masks are requested one group ahead
+      uint32_t ahead = a.vis[t * TILE + tid < n ? t * TILE + tid : 0u];
       for (int j = 0; j < ITEMS; ++j) {
         const uint32_t base = j * THREADS;
-        const uint32_t all = t * TILE + base + tid < n ? a.vis[t * TILE + base + tid] : 0u;
+        const uint32_t all = t * TILE + base + tid < n ? ahead : 0u;
+        if (j + 1 < ITEMS) {
+          const uint32_t nx = t * TILE + base + THREADS + tid;
+          ahead = a.vis[nx < n ? nx : 0u];
+        }
         const uint32_t mine = (all >> v0) & ((1u << nh) - 1u);
         uint32_t total;
         uint32_t slot = block_exclusive_scan<THREADS>((uint32_t)__popc(mine), s_tmp, total);
