@@ -238,14 +238,26 @@ __global__ __launch_bounds__(1024) void k_tile_order(OrderLaunch a) {
   __syncthreads();
   const uint32_t chunk = (T + THREADS - 1) / THREADS;
   const uint32_t t0 = min(tid * chunk, T), t1 = min(t0 + chunk, T);
-  auto cls = [&](uint32_t t) -> int {  // log2 class of a heavy candidate, -1 otherwise
-    const uint2 r = ranges[t];
-    const uint32_t len = r.y - r.x;
-    return len >= thr ? 31 - __clz(len) : -1;
+  // log2 classes of a heavy candidate (-1 otherwise) for the eight tiles
+  // from tb on, past t1 -2: their ranges are requested together (past the
+  // end a lane reads tile 0's and drops it) instead of one after another
+  constexpr uint32_t G = 8;
+  auto classes = [&](uint32_t tb, int (&c)[G]) {
+    uint2 r[G];
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g) r[g] = ranges[tb + g < t1 ? tb + g : 0u];
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g) {
+      const uint32_t len = r[g].y - r[g].x;
+      c[g] = tb + g >= t1 ? -2 : (len >= thr ? 31 - __clz(len) : -1);
+    }
   };
-  for (uint32_t t = t0; t < t1; ++t) {
-    const int c = cls(t);
-    if (c >= 0) atomicAdd(&s_cnt[c], 1u);
+  for (uint32_t tb = t0; tb < t1; tb += G) {
+    int c[G];
+    classes(tb, c);
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g)
+      if (c[g] >= 0) atomicAdd(&s_cnt[c[g]], 1u);
   }
   __syncthreads();
   if (tid == 0) {
@@ -269,16 +281,26 @@ __global__ __launch_bounds__(1024) void k_tile_order(OrderLaunch a) {
   __syncthreads();
   const uint32_t cut = s_cut;
   uint32_t light = 0;
-  for (uint32_t t = t0; t < t1; ++t) {
-    const int c = cls(t);
-    if (c >= (int)cut) order[atomicAdd(&s_cur[c], 1u)] = t;
-    else ++light;
+  for (uint32_t tb = t0; tb < t1; tb += G) {
+    int c[G];
+    classes(tb, c);
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g) {
+      if (c[g] == -2) continue;
+      if (c[g] >= (int)cut) order[atomicAdd(&s_cur[c[g]], 1u)] = tb + g;
+      else ++light;
+    }
   }
   uint32_t total;
   uint32_t pos = block_exclusive_scan<THREADS>(light, s_tmp, total);
   const uint32_t heavy = T - total;
-  for (uint32_t t = t0; t < t1; ++t)
-    if (cls(t) < (int)cut) order[heavy + pos++] = t;
+  for (uint32_t tb = t0; tb < t1; tb += G) {
+    int c[G];
+    classes(tb, c);
+#pragma unroll
+    for (uint32_t g = 0; g < G; ++g)
+      if (c[g] != -2 && c[g] < (int)cut) order[heavy + pos++] = tb + g;
+  }
 }
 
 void launch_tile_order(const OrderLaunch& a, hipStream_t s) {
