@@ -358,7 +358,7 @@ void launch_sequence(spl_renderer* r, const BatchShape& b, hipStream_t q, hipEve
     a.passes = P;
     a.bits = bits;
     a.clear = r->cull_status; a.clear_words = cull_used; a.clear_stride = r->cull_tiles;
-    a.grid_x = clampg(div_up(n, 256 * 16), per_view);
+    a.grid_x = clampg(div_up(n, 256 * 64), per_view);  // 64+ keys a lane: fewer blocks flushing bins
     a.views = (int)nv;
     launch_histograms(a, q);
   }
