@@ -65,6 +65,9 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   __shared__ uint32_t s_org[TILE];
   __shared__ uint32_t s_w[TILE];
   __shared__ uint32_t s_rec[TILE];
+  // the entry that owns each output slot of a tile with at most OWN pairs
+  constexpr uint32_t OWN = 4096, PER = OWN / THREADS;
+  __shared__ uint16_t s_own[OWN];
   __shared__ uint32_t s_hist[kMaxPairPasses * 256];
   __shared__ uint32_t s_tmp[WAVES];
   __shared__ uint32_t s_tile;
@@ -152,11 +155,60 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
     }
     __syncthreads();
     const uint64_t prefix = s_prefix;
+    // Owner of each output slot: every entry with pairs writes its index at
+    // its first slot, then an inclusive max-scan over the slots carries it
+    // forward, so the expansion reads a slot's owner instead of
+    // binary-searching the offsets (ten dependent LDS reads).  A tile with
+    // more than OWN pairs searches.
+    const bool table = total <= OWN;
+    if (table) {
+      const uint32_t lane = tid & (kWave - 1), wv = tid / kWave;
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t q = tid * PER + k;
+        if (q < total) s_own[q] = 0;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t j = tid * ITEMS + k;
+        const uint32_t st = s_off[j];
+        if (s_off[j + 1] > st) s_own[st] = (uint16_t)j;
+      }
+      __syncthreads();
+      uint32_t run = 0;
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t q = tid * PER + k;
+        if (q < total) run = max(run, (uint32_t)s_own[q]);
+      }
+      uint32_t incl = run;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= (uint32_t)o) incl = max(incl, y);
+      }
+      if (lane == kWave - 1) s_tmp[wv] = incl;
+      __syncthreads();
+      uint32_t carry = __shfl_up(incl, 1, kWave);
+      if (lane == 0) carry = 0;
+      for (uint32_t i = 0; i < wv; ++i) carry = max(carry, s_tmp[i]);
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t q = tid * PER + k;
+        if (q < total) {
+          carry = max(carry, (uint32_t)s_own[q]);
+          s_own[q] = (uint16_t)carry;
+        }
+      }
+      __syncthreads();
+    }
     for (uint32_t s = tid; s < total; s += THREADS) {
       uint32_t lo = 0, hi = TILE - 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (s_off[mid] <= s) lo = mid; else hi = mid - 1;
+      if (table) {
+        lo = s_own[s];
+      } else {
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (s_off[mid] <= s) lo = mid; else hi = mid - 1;
+        }
       }
       const uint64_t pos = prefix + s;
       if (pos < a.pair_cap) {
