@@ -340,8 +340,8 @@ def copy_calibration(nbytes: int = 256 << 20, repeats: int = 3) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=48)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--splats", type=int, default=6_000_000)
     ap.add_argument("--views", type=int, default=16)
     ap.add_argument("--width", type=int, default=1920)
