@@ -356,7 +356,7 @@ def main() -> int:
                     help="batches in flight: the batched loop alternates this many renderers, each with its own "
                          "frames, so one batch's raster overlaps the next one's sort and projection")
     ap.add_argument("--serving-rate", type=float, default=200.0, help="Poisson arrivals per second")
-    ap.add_argument("--serving-rate-high", type=float, default=1500.0,
+    ap.add_argument("--serving-rate-high", type=float, default=1800.0,
                     help="a second Poisson rate, high enough that requests queue and batches form (0 skips it)")
     ap.add_argument("--serving-seconds", type=float, default=8.0, help="0 skips the serving measurement")
     ap.add_argument("--serving-sweep", type=str, default="",
