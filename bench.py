@@ -437,9 +437,12 @@ def main() -> int:
     poses = orbit_poses(256)
     if args.tune_only:
         # radix digit width x sort tile (block shape) at this batch width,
-        # each candidate timed on the same resident scene
+        # each candidate timed on the same resident scene over the whole
+        # orbit, as the timed steps are (the poses differ in cost, so part
+        # of the orbit is a different workload)
         pipe[0].close()
-        res = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER, steps=8, log=log,
+        res = at.autotune_staged(sa, dev, cfg, lambda _r: scene, poses, W, H, V_PER,
+                                 steps=max(1, len(poses) // V_PER), log=log,
                                  in_flight=max(1, args.in_flight))
         print(json.dumps(res.to_json()), flush=True)
         return 0
