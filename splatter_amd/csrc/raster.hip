@@ -55,6 +55,281 @@ __device__ __forceinline__ float q8(float x) {
 
 __device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
+// Pixel centres in the view plane are q = (sx px + ox, sy py + oy)
+// (shaders.wgsl:561 inverted).  Each staged entry folds this map into its
+// own: with e = o - t the coverage coordinate is affine in the integer
+// pixel coordinates,
+//   unaligned:     u = au px + bu py + cu,  w = av px + bv py + cv
+//   axis-aligned:  dx = sx px + ex,  dy = sy py + ey  (then u, w from dx, dy)
+// so a pixel costs two fused products per coordinate.
+struct PixelMap {
+  float sx, ox, sy, oy;
+};
+
+__device__ __forceinline__ PixelMap pixel_map(const Uniforms& u) {
+  const float Wf = (float)u.image_size[0], Hf = (float)u.image_size[1], vw = u.view_size[0], vh = u.view_size[1];
+  PixelMap p;
+  p.sx = (2.0f * vw) / Wf;
+  p.ox = vw / Wf - vw;
+  p.sy = -(2.0f * vh) / Hf;
+  p.oy = vh - vh / Hf;
+  return p;
+}
+
+// The staged form of a record (g0: colour and opacity, g1: the rectangle's
+// axes, g2: its centre and pixel box): e0 the inverse map scaled by S (and,
+// unaligned, folded with the pixel map), e1 its translation, the coverage
+// half-extent and the opacity, e2 the colour.
+template <bool UNALIGNED>
+__device__ __forceinline__ void prepare_entry(const float4& g0, const float4& g1, const float4& g2, const PixelMap& pm,
+                                              float margin, float4& e0, float4& e1, float4& e2) {
+  const float A00 = g1.x, A10 = g1.y, A01 = g1.z, A11 = g1.w;
+  const float det = A00 * A11 - A10 * A01;
+  const float rs = (1.0f / det) * kGaussScale;  // inverse map scaled by S
+  const float i00 = A11 * rs, i01 = -A01 * rs, i10 = -A10 * rs, i11 = A00 * rs;
+  const float ex = pm.ox - g2.x, ey = pm.oy - g2.y;
+  // unaligned: the rectangle |tc| < margin in scaled coordinates;
+  // axis-aligned: a square of half-side radius * margin in the view plane
+  const float half = UNALIGNED ? margin * kGaussScale
+                               : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
+  if constexpr (UNALIGNED) {
+    e0 = make_float4(i00 * pm.sx, i01 * pm.sy, i10 * pm.sx, i11 * pm.sy);
+    e1 = make_float4(fmaf(i00, ex, i01 * ey), fmaf(i10, ex, i11 * ey), half, g0.w);
+  } else {
+    e0 = make_float4(i00, i01, i10, i11);
+    e1 = make_float4(ex, ey, half, g0.w);
+  }
+  e2 = make_float4(g0.x, g0.y, g0.z, 0.0f);
+}
+
+// Can entry (m, t) cover a pixel centre of the pixel box [x0, x1] x
+// [y0, y1]?  The tested coordinates are affine in the pixel, so their exact
+// range over the box sits at the corners; the slack (relative to the
+// magnitudes the per-pixel evaluation adds up) is far above its rounding.
+template <bool UNALIGNED>
+__device__ __forceinline__ bool range_meets(const float4& m, const float4& t, float x0, float x1, float y0, float y1,
+                                            const PixelMap& pm) {
+  const float h = t.z;
+  float a0, a1, b0, b1, sa, sb;  // ranges of the tested coordinates, and slack
+  if constexpr (UNALIGNED) {
+    a0 = (fminf(m.x * x0, m.x * x1) + fminf(m.y * y0, m.y * y1)) + t.x;
+    a1 = (fmaxf(m.x * x0, m.x * x1) + fmaxf(m.y * y0, m.y * y1)) + t.x;
+    b0 = (fminf(m.z * x0, m.z * x1) + fminf(m.w * y0, m.w * y1)) + t.y;
+    b1 = (fmaxf(m.z * x0, m.z * x1) + fmaxf(m.w * y0, m.w * y1)) + t.y;
+    sa = 1e-4f * (((fabsf(m.x) * x1 + fabsf(m.y) * y1) + fabsf(t.x)) + h);
+    sb = 1e-4f * (((fabsf(m.z) * x1 + fabsf(m.w) * y1) + fabsf(t.y)) + h);
+  } else {
+    a0 = fmaf(pm.sx, x0, t.x); a1 = fmaf(pm.sx, x1, t.x);  // sx > 0
+    b0 = fmaf(pm.sy, y1, t.y); b1 = fmaf(pm.sy, y0, t.y);  // sy < 0
+    sa = 1e-4f * ((pm.sx * x1 + fabsf(t.x)) + h);
+    sb = 1e-4f * ((fabsf(pm.sy) * y1 + fabsf(t.y)) + h);
+  }
+  return a1 + sa >= -h && a0 - sa < h && b1 + sb >= -h && b0 - sb < h;
+}
+
+// Coverage of the pixel centre (pxf, pyf) by entry (m, t): the scaled
+// texture coordinate (u, w2) and whether it lies in the rectangle.
+template <bool UNALIGNED>
+__device__ __forceinline__ bool covers(const float4& m, const float4& t, float pxf, float pyf, const PixelMap& pm,
+                                       float& u, float& w2) {
+  const float h = t.z;
+  if constexpr (UNALIGNED) {
+    u = fmaf(m.x, pxf, fmaf(m.y, pyf, t.x));  // S * texture coordinate
+    w2 = fmaf(m.z, pxf, fmaf(m.w, pyf, t.y));
+    return fmaxf(fabsf(u), fabsf(w2)) < h;
+  } else {
+    const float dx = fmaf(pm.sx, pxf, t.x), dy = fmaf(pm.sy, pyf, t.y);
+    u = fmaf(m.x, dx, m.y * dy);
+    w2 = fmaf(m.z, dx, m.w * dy);
+    return fmaxf(fabsf(dx), fabsf(dy)) < h;
+  }
+}
+
+// One pixel's blend state: colour and transmittance, unorm8 codes or f32.
+template <bool F32>
+struct PixelState {
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = F32 ? 1.0f : 255.0f;
+
+  // One fragment: alpha (shaders.wgsl:585-589) and the blend.  in: the
+  // coverage test's verdict, for walks over candidates that did not pass
+  // it yet; an uncovered fragment blends nothing.
+  __device__ __forceinline__ void shade(float u, float w2, float opacity, const float4& col, bool in) {
+    float alpha = opacity * spl_exp2(-fmaf(u, u, w2 * w2));
+    if constexpr (!F32) {
+      // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
+      // 255) and rint(T * (1 - a)): the ideal 8-bit blender.  Below the
+      // threshold alpha becomes 0, which leaves C and T as they are
+      // (they hold integers; clamp01 of a NaN product is 0), so no
+      // branch is needed.
+      alpha = in && alpha >= 1.0f / 255.0f ? alpha : 0.0f;
+      const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
+      // the source alpha needs no clamp: the opacity is a sigmoid (or 1
+      // in the dev-test scene) and spl_exp2 of a non-positive argument is
+      // at most 1 (p(f) <= p(0) = 1 when rint(y) = 0, p(f) / 2 < 1 below),
+      // so alpha is in [0, 1] after the threshold
+      const float sa = alpha;
+      c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
+      c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
+      c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
+      tr = rintf(fmaf(-sa, tr, tr));
+    } else if (in && alpha >= 1.0f / 255.0f) {
+      const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
+      const float da = tr;
+      c0 = fmaf(sr, da, c0);
+      c1 = fmaf(sg, da, c1);
+      c2 = fmaf(sb, da, c2);
+      tr = fmaf(-alpha, da, da);
+    }
+  }
+
+  // The pixel's final value, as the frame stores it.
+  __device__ __forceinline__ uint32_t packed(bool bgra) const {
+    uint32_t r, g, b, al;
+    if (!F32) {
+      r = (uint32_t)c0; g = (uint32_t)c1; b = (uint32_t)c2; al = (uint32_t)tr;
+    } else {
+      r = (uint32_t)q8(c0); g = (uint32_t)q8(c1); b = (uint32_t)q8(c2); al = (uint32_t)q8(tr);
+    }
+    return bgra ? (b | (g << 8) | (r << 16) | (al << 24)) : (r | (g << 8) | (b << 16) | (al << 24));
+  }
+};
+
+// Per-lane walk over one run of cnt <= 32 entries (cm, ct, cc: their three
+// staged rows, in list order, in LDS) of a wave's 8x8 region with top-left
+// pixel (fqx0, fqy0); rows: 64 words of LDS for the coverage rows.  The wave
+// first finds, for every pixel, the run's entries that can cover it, then
+// every lane walks only those, in list order, each lane on its own: the
+// wave takes as many steps as its busiest pixel has covering entries, not
+// as many as the run has.
+//
+// The candidates come per row, entry-parallel: 2, 4 or 8 lanes per entry
+// each take 4, 2 or 1 of the region's rows and solve the rectangle's two
+// slabs |a px + b py + c| < h for the row's column interval, widened by far
+// more than the rounding of either that solution or the per-pixel test
+// (2^-19 of the magnitudes, against about 12 roundings of 2^-24), so it
+// holds every column whose centre the per-pixel test accepts.  A slab that
+// hardly varies along x (|a| < h 1e-6), or with a coefficient that is not
+// finite, is taken as all columns.  Each pixel lane then gathers its row's
+// 32 bits from LDS (row y: byte 3 - (k >> 3) of word k & 7 holds slot k's
+// row mask, bit x = column x), and the walk applies the exact per-pixel
+// test of the other modes to every candidate (a candidate that fails it
+// blends with alpha 0, which leaves the pixel as it is), so the pixels are
+// the same.  Runs of up to 12 entries are tested per pixel instead.
+template <bool UNALIGNED, bool F32>
+__device__ __forceinline__ void walk_run(const float4* cm, const float4* ct, const float4* cc, uint32_t* rows,
+                                         uint32_t cnt, uint32_t lane, bool inside, float pxf, float pyf, float fqx0,
+                                         float fqy0, float span, const PixelMap& pm, float margin,
+                                         PixelState<F32>& pix) {
+  uint8_t* const rows8 = reinterpret_cast<uint8_t*>(rows);
+  constexpr uint32_t kRowsMin = 12;  // runs up to this long are tested per pixel
+  // column interval of row y (local, 0..7) as [ceil(lo), floor(hi1) - 1],
+  // clamped to the region, as a row mask
+  auto row_mask = [&](float lo, float hi1) -> uint32_t {
+    const uint32_t first = (uint32_t)__builtin_amdgcn_fmed3f(ceilf(lo), 0.0f, 8.0f);
+    const uint32_t end = (uint32_t)__builtin_amdgcn_fmed3f(floorf(hi1), 0.0f, 8.0f);
+    return ((1u << end) - 1u) & ~((1u << first) - 1u);
+  };
+  // slab |a px + b py + c| < h in local columns: lo(y) = P y + lo0,
+  // hi(y) + 1 = P y + hi0
+  auto slab = [&](float a_, float b_, float c_, float h_, float& P, float& lo0, float& hi0) {
+    const float ia = __builtin_amdgcn_rcpf(a_);
+    const float R = h_ * fabsf(ia);
+    const float Pv = -b_ * ia;
+    const float Qg = -c_ * ia;
+    const float Q = fmaf(Pv, fqy0, Qg) - fqx0;
+    const float eps = 0x1p-19f * (((fabsf(Pv) * span + fabsf(Qg)) + R) + span) + 0x1p-19f;
+    // (R = 0 for an infinite a, whose products with a zero
+    // coordinate are NaN in the per-pixel test)
+    const bool ok = R > 0.0f && R < 1e6f && fabsf(Pv) < 1e6f && fabsf(Qg) < 1e9f;
+    P = ok ? Pv : 0.0f;
+    lo0 = ok ? (Q - R) - eps : -1e9f;
+    hi0 = ok ? ((Q + R) + eps) + 1.0f : 1e9f;
+  };
+  uint32_t mask = 0;
+  if (cnt <= kRowsMin) {
+    // a short run: every lane tests its pixel against each slot
+    // (exact), slot k at bit K - 1 - k for K = cnt rounded up to 4
+    for (uint32_t k = 0; k < cnt; k += 4) {
+      uint32_t nib = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float u, w2;
+        float4 t4;
+        if constexpr (UNALIGNED) {
+          // the half-extent is margin * S for every entry
+          const float2 t2 = *reinterpret_cast<const float2*>(&ct[k + q]);
+          t4 = make_float4(t2.x, t2.y, margin * kGaussScale, 0.0f);
+        } else {
+          t4 = ct[k + q];
+        }
+        nib = (nib << 1) | (covers<UNALIGNED>(cm[k + q], t4, pxf, pyf, pm, u, w2) ? 1u : 0u);
+      }
+      mask = (mask << 4) | nib;
+    }
+    mask <<= 32u - ((cnt + 3u) & ~3u);
+  } else {
+    {
+      const uint32_t lpe = cnt <= 8u ? 3u : (cnt <= 16u ? 2u : 1u);  // log2 lanes per entry
+      const uint32_t k = lane >> lpe;
+      if (k < cnt) {
+        const uint32_t rpl = 8u >> lpe;  // rows per lane
+        const uint32_t y0 = (lane & ((1u << lpe) - 1u)) * rpl;
+        const float4 m = cm[k];
+        const float4 t = ct[k];
+        uint8_t* const dst = rows8 + (k & 7u) * 4u + (3u - (k >> 3));
+        if constexpr (UNALIGNED) {
+          const float h = margin * kGaussScale;
+          float P1, l1, e1, P2, l2, e2;
+          slab(m.x, m.y, t.x, h, P1, l1, e1);
+          slab(m.z, m.w, t.y, h, P2, l2, e2);
+          for (uint32_t i = 0; i < rpl; ++i) {
+            const float y = (float)(y0 + i);
+            const float lo = fmaxf(fmaf(P1, y, l1), fmaf(P2, y, l2));
+            const float hi1 = fminf(fmaf(P1, y, e1), fmaf(P2, y, e2));
+            dst[(y0 + i) * 32u] = (uint8_t)row_mask(lo, hi1);
+          }
+        } else {
+          // dx = sx px + t.x bounds the columns, dy = sy py + t.y the rows
+          float P, lx, ex, Py, ly, ey;
+          slab(pm.sx, 0.0f, t.x, t.z, P, lx, ex);
+          slab(pm.sy, 0.0f, t.y, t.z, Py, ly, ey);
+          // rows: the same interval, in rows (py in place of px)
+          ly += fqx0 - fqy0;
+          ey += fqx0 - fqy0;
+          const uint32_t cols = row_mask(lx, ex);
+          const uint32_t rmask = row_mask(ly, ey);
+          for (uint32_t i = 0; i < rpl; ++i)
+            dst[(y0 + i) * 32u] = (uint8_t)(((rmask >> (y0 + i)) & 1u) ? cols : 0u);
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // this pixel's candidates: slot k at bit 31 - k
+    const uint32_t xl = lane & 7u;
+    const uint4 ra = *reinterpret_cast<const uint4*>(&rows[(lane >> 3) * 8u]);
+    const uint4 rb = *reinterpret_cast<const uint4*>(&rows[(lane >> 3) * 8u + 4u]);
+    mask = (mask << 1) | ((ra.x >> xl) & 0x01010101u);
+    mask = (mask << 1) | ((ra.y >> xl) & 0x01010101u);
+    mask = (mask << 1) | ((ra.z >> xl) & 0x01010101u);
+    mask = (mask << 1) | ((ra.w >> xl) & 0x01010101u);
+    mask = (mask << 1) | ((rb.x >> xl) & 0x01010101u);
+    mask = (mask << 1) | ((rb.y >> xl) & 0x01010101u);
+    mask = (mask << 1) | ((rb.z >> xl) & 0x01010101u);
+    mask = (mask << 1) | ((rb.w >> xl) & 0x01010101u);
+  }
+  mask &= ~0u << (32u - cnt);  // slots at or past cnt hold stale rows
+  if (pix.tr == 0.0f || !inside) mask = 0u;
+  while (mask) {
+    const uint32_t k = (uint32_t)__builtin_clz(mask);
+    mask &= ~(0x80000000u >> k);
+    const float4 t = ct[k];
+    float u, w2;
+    const bool in = covers<UNALIGNED>(cm[k], t, pxf, pyf, pm, u, w2);
+    pix.shade(u, w2, t.w, cc[k], in);
+    if (pix.tr == 0.0f) mask = 0u;
+  }
+}
+
 }  // namespace
 
 template <int THREADS, int ITEMS>
@@ -452,43 +727,10 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
   // (the ranges are cleared for the next frame by its pair sort)
   const uint2 range = ranges[tile];
 
-  // Pixel centres in the view plane are q = (sx px + ox, sy py + oy)
-  // (shaders.wgsl:561 inverted).  Each staged entry folds this map into its
-  // own: with e = o - t the coverage coordinate is affine in the integer
-  // pixel coordinates,
-  //   unaligned:     u = au px + bu py + cu,  w = av px + bv py + cv
-  //   axis-aligned:  dx = sx px + ex,  dy = sy py + ey  (then u, w from dx, dy)
-  // so a pixel costs two fused products per coordinate.
-  const float Wf = (float)W, Hf = (float)H, vw = u.view_size[0], vh = u.view_size[1];
-  const float sx = (2.0f * vw) / Wf, ox = vw / Wf - vw;
-  const float sy = -(2.0f * vh) / Hf, oy = vh - vh / Hf;
+  const PixelMap pm = pixel_map(u);
   const float pxf = (float)px, pyf = (float)py;
-  constexpr bool f32 = F32;
-  constexpr bool unaligned = UNALIGNED;
-  // Can entry (m, t) cover a pixel centre of the pixel box [x0, x1] x
-  // [y0, y1]?  The tested coordinates are affine in the pixel, so their exact
-  // range over the box sits at the corners; the slack (relative to the
-  // magnitudes the per-pixel evaluation adds up) is far above its rounding.
-  auto range_meets = [&](const float4& m, const float4& t, float x0, float x1, float y0, float y1) -> bool {
-    const float h = t.z;
-    float a0, a1, b0, b1, sa, sb;  // ranges of the tested coordinates, and slack
-    if constexpr (unaligned) {
-      a0 = (fminf(m.x * x0, m.x * x1) + fminf(m.y * y0, m.y * y1)) + t.x;
-      a1 = (fmaxf(m.x * x0, m.x * x1) + fmaxf(m.y * y0, m.y * y1)) + t.x;
-      b0 = (fminf(m.z * x0, m.z * x1) + fminf(m.w * y0, m.w * y1)) + t.y;
-      b1 = (fmaxf(m.z * x0, m.z * x1) + fmaxf(m.w * y0, m.w * y1)) + t.y;
-      sa = 1e-4f * (((fabsf(m.x) * x1 + fabsf(m.y) * y1) + fabsf(t.x)) + h);
-      sb = 1e-4f * (((fabsf(m.z) * x1 + fabsf(m.w) * y1) + fabsf(t.y)) + h);
-    } else {
-      a0 = fmaf(sx, x0, t.x); a1 = fmaf(sx, x1, t.x);  // sx > 0
-      b0 = fmaf(sy, y1, t.y); b1 = fmaf(sy, y0, t.y);  // sy < 0
-      sa = 1e-4f * ((sx * x1 + fabsf(t.x)) + h);
-      sb = 1e-4f * ((fabsf(sy) * y1 + fabsf(t.y)) + h);
-    }
-    return a1 + sa >= -h && a0 - sa < h && b1 + sb >= -h && b0 - sb < h;
-  };
   const float margin = u.ellipse_margin;
-  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, tr = f32 ? 1.0f : 255.0f;
+  PixelState<F32> pix;
   const uint32_t visible = a.counters[(size_t)v * kCounterWords + kVisible];
 
   // register stage of one entry per lane
@@ -504,23 +746,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
   };
   auto stage = [&](int buf) {
     if (tid >= (uint32_t)B) return;
-    const float A00 = g1.x, A10 = g1.y, A01 = g1.z, A11 = g1.w;
-    const float det = A00 * A11 - A10 * A01;
-    const float rs = (1.0f / det) * kGaussScale;  // inverse map scaled by S
-    const float i00 = A11 * rs, i01 = -A01 * rs, i10 = -A10 * rs, i11 = A00 * rs;
-    const float ex = ox - g2.x, ey = oy - g2.y;
-    // unaligned: the rectangle |tc| < margin in scaled coordinates;
-    // axis-aligned: a square of half-side radius * margin in the view plane
-    const float half = unaligned ? margin * kGaussScale
-                                 : sqrtf(fmaxf(A00 * A00 + A10 * A10, A01 * A01 + A11 * A11)) * margin;
-    if constexpr (unaligned) {
-      s_ent[buf][0][tid] = make_float4(i00 * sx, i01 * sy, i10 * sx, i11 * sy);
-      s_ent[buf][1][tid] = make_float4(fmaf(i00, ex, i01 * ey), fmaf(i10, ex, i11 * ey), half, g0.w);
-    } else {
-      s_ent[buf][0][tid] = make_float4(i00, i01, i10, i11);
-      s_ent[buf][1][tid] = make_float4(ex, ey, half, g0.w);
-    }
-    s_ent[buf][2][tid] = make_float4(g0.x, g0.y, g0.z, 0.0f);
+    prepare_entry<UNALIGNED>(g0, g1, g2, pm, margin, s_ent[buf][0][tid], s_ent[buf][1][tid], s_ent[buf][2][tid]);
     s_box[buf][tid] = make_uint2(__float_as_uint(g2.z), __float_as_uint(g2.w));
   };
 
@@ -572,7 +798,8 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
           const uint32_t x0 = bx.x & 0xFFFFu, y0 = bx.x >> 16, x1 = bx.y & 0xFFFFu, y1 = bx.y >> 16;
           ov = bx.x != 0xFFFFFFFFu && x0 <= ax1 && x1 >= ax0 && y0 <= ay1 && y1 >= ay0;
           if (ov && !SUBS)
-            ov = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], (float)ax0, (float)ax1, (float)ay0, (float)ay1);
+            ov = range_meets<UNALIGNED>(s_ent[buf][0][j], s_ent[buf][1][j], (float)ax0, (float)ax1, (float)ay0,
+                                        (float)ay1, pm);
         }
         if constexpr (!SUBS) {
           const uint64_t bm = __ballot(ov);
@@ -591,7 +818,8 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
                   (bx.x >> 16) <= by0 + 3u && (bx.y >> 16) >= by0;
               if (o) {
                 const uint32_t bx1 = min(bx0 + 3u, W - 1u), by1 = min(by0 + 3u, H - 1u);
-                o = range_meets(s_ent[buf][0][j], s_ent[buf][1][j], (float)bx0, (float)bx1, (float)by0, (float)by1);
+                o = range_meets<UNALIGNED>(s_ent[buf][0][j], s_ent[buf][1][j], (float)bx0, (float)bx1, (float)by0,
+                                           (float)by1, pm);
               }
             }
             const uint64_t bm = __ballot(o);
@@ -601,51 +829,14 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
         }
       }
       __builtin_amdgcn_wave_barrier();
-      // one fragment of entry slot j at this lane's pixel: coverage, alpha
-      // (shaders.wgsl:585-589) and the blend; identical arithmetic on both
-      // walks below, so they give identical pixels
-      auto covers = [&](const float4& m, const float4& t, float& u, float& w2) -> bool {
-        const float h = t.z;
-        if constexpr (unaligned) {
-          u = fmaf(m.x, pxf, fmaf(m.y, pyf, t.x));  // S * texture coordinate
-          w2 = fmaf(m.z, pxf, fmaf(m.w, pyf, t.y));
-          return fmaxf(fabsf(u), fabsf(w2)) < h;
-        } else {
-          const float dx = fmaf(sx, pxf, t.x), dy = fmaf(sy, pyf, t.y);
-          u = fmaf(m.x, dx, m.y * dy);
-          w2 = fmaf(m.z, dx, m.w * dy);
-          return fmaxf(fabsf(dx), fabsf(dy)) < h;
-        }
+      // one fragment at this lane's pixel: coverage, alpha (shaders.wgsl:585-589)
+      // and the blend; identical arithmetic on every walk, so they give
+      // identical pixels
+      auto covers_ = [&](const float4& m, const float4& t, float& u, float& w2) -> bool {
+        return covers<UNALIGNED>(m, t, pxf, pyf, pm, u, w2);
       };
-      // (in: the coverage test's verdict, for walks over candidates that
-      // did not pass it yet; an uncovered fragment blends nothing)
       auto shade = [&](float u, float w2, float opacity, const float4& col, bool in) {
-        float alpha = opacity * spl_exp2(-fmaf(u, u, w2 * w2));
-        if constexpr (!f32) {
-          // unorm8 codes C, T: rint(s * T + C) (s clamped, sum capped at
-          // 255) and rint(T * (1 - a)): the ideal 8-bit blender.  Below the
-          // threshold alpha becomes 0, which leaves C and T as they are
-          // (they hold integers; clamp01 of a NaN product is 0), so no
-          // branch is needed.
-          alpha = in && alpha >= 1.0f / 255.0f ? alpha : 0.0f;
-          const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
-          // the source alpha needs no clamp: the opacity is a sigmoid (or 1
-          // in the dev-test scene) and spl_exp2 of a non-positive argument is
-          // at most 1 (p(f) <= p(0) = 1 when rint(y) = 0, p(f) / 2 < 1 below),
-          // so alpha is in [0, 1] after the threshold
-          const float sa = alpha;
-          c0 = fminf(rintf(fmaf(clamp01(sr), tr, c0)), 255.0f);
-          c1 = fminf(rintf(fmaf(clamp01(sg), tr, c1)), 255.0f);
-          c2 = fminf(rintf(fmaf(clamp01(sb), tr, c2)), 255.0f);
-          tr = rintf(fmaf(-sa, tr, tr));
-        } else if (in && alpha >= 1.0f / 255.0f) {
-          const float sr = col.x * alpha, sg = col.y * alpha, sb = col.z * alpha;
-          const float da = tr;
-          c0 = fmaf(sr, da, c0);
-          c1 = fmaf(sg, da, c1);
-          c2 = fmaf(sb, da, c2);
-          tr = fmaf(-alpha, da, da);
-        }
+        pix.shade(u, w2, opacity, col, in);
       };
       if constexpr (SUBS) {
         // entry-major per 4x4 sub-block: the four lane groups step through
@@ -670,61 +861,19 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
 #pragma unroll
           for (int k = 0; k < U; ++k) {
             float u, w2;
-            if (i + k < mine && covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k], true);
+            if (i + k < mine && covers_(m[k], t[k], u, w2) && pix.tr != 0.0f) shade(u, w2, t[k].w, col[k], true);
           }
         }
       } else if constexpr (LANES) {
         // per-lane walk: for 32 list slots at a time the wave first copies
         // the slots' entries into a compact run (its 32 entries of the
         // other staging buffer, free until the next batch is staged), then
-        // finds, for every pixel, the run's entries that can cover it, and
-        // every lane walks only those, in list order, each lane on its own:
-        // the wave takes as many steps as its busiest pixel has covering
-        // entries, not as many as the list has.
-        //
-        // The candidates come per row, entry-parallel: 2, 4 or 8 lanes per
-        // entry each take 4, 2 or 1 of the quadrant's rows and solve the
-        // rectangle's two slabs |a px + b py + c| < h for the row's column
-        // interval, widened by far more than the rounding of either that
-        // solution or the per-pixel test (2^-19 of the magnitudes, against
-        // about 12 roundings of 2^-24), so it holds every column whose
-        // centre the per-pixel test accepts.  A slab that hardly varies
-        // along x (|a| < h 1e-6), or with a coefficient that is not finite,
-        // is taken as all columns.  Each pixel lane
-        // then gathers its row's 32 bits from LDS, and the walk applies the
-        // exact per-pixel test of the other modes to every candidate (a
-        // candidate that fails it blends with alpha 0, which leaves the
-        // pixel as it is), so the pixels are the same.
+        // walks the run (walk_run: each pixel only its covering entries)
         float4* const cm = &s_ent[buf ^ 1][0][32 * w];
         float4* const ct = &s_ent[buf ^ 1][1][32 * w];
         float4* const cc = &s_ent[buf ^ 1][2][32 * w];
-        uint8_t* const rows8 = reinterpret_cast<uint8_t*>(&s_rows[w][0]);
-        constexpr uint32_t kRowsMin = 12;  // runs up to this long are tested per pixel
         const float fqx0 = (float)qx0, fqy0 = (float)qy0;
-        const float span = Wf + Hf;  // bounds the pixel coordinates' magnitudes
-        // column interval of row y (local, 0..7) as [ceil(lo), floor(hi1) - 1],
-        // clamped to the quadrant, as a row mask
-        auto row_mask = [&](float lo, float hi1) -> uint32_t {
-          const uint32_t first = (uint32_t)__builtin_amdgcn_fmed3f(ceilf(lo), 0.0f, 8.0f);
-          const uint32_t end = (uint32_t)__builtin_amdgcn_fmed3f(floorf(hi1), 0.0f, 8.0f);
-          return ((1u << end) - 1u) & ~((1u << first) - 1u);
-        };
-        // slab |a px + b py + c| < h in local columns: lo(y) = P y + lo0,
-        // hi(y) + 1 = P y + hi0
-        auto slab = [&](float a_, float b_, float c_, float h_, float& P, float& lo0, float& hi0) {
-          const float ia = __builtin_amdgcn_rcpf(a_);
-          const float R = h_ * fabsf(ia);
-          const float Pv = -b_ * ia;
-          const float Qg = -c_ * ia;
-          const float Q = fmaf(Pv, fqy0, Qg) - fqx0;
-          const float eps = 0x1p-19f * (((fabsf(Pv) * span + fabsf(Qg)) + R) + span) + 0x1p-19f;
-          // (R = 0 for an infinite a, whose products with a zero
-          // coordinate are NaN in the per-pixel test)
-          const bool ok = R > 0.0f && R < 1e6f && fabsf(Pv) < 1e6f && fabsf(Qg) < 1e9f;
-          P = ok ? Pv : 0.0f;
-          lo0 = ok ? (Q - R) - eps : -1e9f;
-          hi0 = ok ? ((Q + R) + eps) + 1.0f : 1e9f;
-        };
+        const float span = (float)W + (float)H;  // bounds the pixel coordinates' magnitudes
         for (uint32_t c = 0; c < n; c += 32) {
           const uint32_t cnt = min(32u, n - c);
           if (lane < cnt) {
@@ -734,89 +883,8 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
             cc[lane] = s_ent[buf][2][j];
           }
           __builtin_amdgcn_wave_barrier();
-          uint32_t mask = 0;
-          if (cnt <= kRowsMin) {
-            // a short run: every lane tests its pixel against each slot
-            // (exact), slot k at bit K - 1 - k for K = cnt rounded up to 4
-            for (uint32_t k = 0; k < cnt; k += 4) {
-              uint32_t nib = 0;
-#pragma unroll
-              for (int q = 0; q < 4; ++q) {
-                float u, w2;
-                float4 t4;
-                if constexpr (unaligned) {
-                  // the half-extent is margin * S for every entry
-                  const float2 t2 = *reinterpret_cast<const float2*>(&ct[k + q]);
-                  t4 = make_float4(t2.x, t2.y, margin * kGaussScale, 0.0f);
-                } else {
-                  t4 = ct[k + q];
-                }
-                nib = (nib << 1) | (covers(cm[k + q], t4, u, w2) ? 1u : 0u);
-              }
-              mask = (mask << 4) | nib;
-            }
-            mask <<= 32u - ((cnt + 3u) & ~3u);
-          } else {
-            {
-              const uint32_t lpe = cnt <= 8u ? 3u : (cnt <= 16u ? 2u : 1u);  // log2 lanes per entry
-              const uint32_t k = lane >> lpe;
-              if (k < cnt) {
-                const uint32_t rpl = 8u >> lpe;  // rows per lane
-                const uint32_t y0 = (lane & ((1u << lpe) - 1u)) * rpl;
-                const float4 m = cm[k];
-                const float4 t = ct[k];
-                uint8_t* const dst = rows8 + (k & 7u) * 4u + (3u - (k >> 3));
-                if constexpr (unaligned) {
-                  const float h = margin * kGaussScale;
-                  float P1, l1, e1, P2, l2, e2;
-                  slab(m.x, m.y, t.x, h, P1, l1, e1);
-                  slab(m.z, m.w, t.y, h, P2, l2, e2);
-                  for (uint32_t i = 0; i < rpl; ++i) {
-                    const float y = (float)(y0 + i);
-                    const float lo = fmaxf(fmaf(P1, y, l1), fmaf(P2, y, l2));
-                    const float hi1 = fminf(fmaf(P1, y, e1), fmaf(P2, y, e2));
-                    dst[(y0 + i) * 32u] = (uint8_t)row_mask(lo, hi1);
-                  }
-                } else {
-                  // dx = sx px + t.x bounds the columns, dy = sy py + t.y the rows
-                  float P, lx, ex, Py, ly, ey;
-                  slab(sx, 0.0f, t.x, t.z, P, lx, ex);
-                  slab(sy, 0.0f, t.y, t.z, Py, ly, ey);
-                  // rows: the same interval, in rows (py in place of px)
-                  ly += fqx0 - fqy0;
-                  ey += fqx0 - fqy0;
-                  const uint32_t cols = row_mask(lx, ex);
-                  const uint32_t rmask = row_mask(ly, ey);
-                  for (uint32_t i = 0; i < rpl; ++i)
-                    dst[(y0 + i) * 32u] = (uint8_t)(((rmask >> (y0 + i)) & 1u) ? cols : 0u);
-                }
-              }
-            }
-            __builtin_amdgcn_wave_barrier();
-            // this pixel's candidates: slot k at bit 31 - k
-            const uint32_t xl = lane & 7u;
-            const uint4 ra = *reinterpret_cast<const uint4*>(&s_rows[w][(lane >> 3) * 8u]);
-            const uint4 rb = *reinterpret_cast<const uint4*>(&s_rows[w][(lane >> 3) * 8u + 4u]);
-            mask = (mask << 1) | ((ra.x >> xl) & 0x01010101u);
-            mask = (mask << 1) | ((ra.y >> xl) & 0x01010101u);
-            mask = (mask << 1) | ((ra.z >> xl) & 0x01010101u);
-            mask = (mask << 1) | ((ra.w >> xl) & 0x01010101u);
-            mask = (mask << 1) | ((rb.x >> xl) & 0x01010101u);
-            mask = (mask << 1) | ((rb.y >> xl) & 0x01010101u);
-            mask = (mask << 1) | ((rb.z >> xl) & 0x01010101u);
-            mask = (mask << 1) | ((rb.w >> xl) & 0x01010101u);
-          }
-          mask &= ~0u << (32u - cnt);  // slots at or past cnt hold stale rows
-          if (tr == 0.0f || !inside) mask = 0u;
-          while (mask) {
-            const uint32_t k = (uint32_t)__builtin_clz(mask);
-            mask &= ~(0x80000000u >> k);
-            const float4 t = ct[k];
-            float u, w2;
-            const bool in = covers(cm[k], t, u, w2);
-            shade(u, w2, t.w, cc[k], in);
-            if (tr == 0.0f) mask = 0u;
-          }
+          walk_run<UNALIGNED, F32>(cm, ct, cc, &s_rows[w][0], cnt, lane, inside, pxf, pyf, fqx0, fqy0, span, pm,
+                                   margin, pix);
           __builtin_amdgcn_wave_barrier();
         }
       } else if constexpr (!WALK) {
@@ -841,7 +909,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
           for (int k = 0; k < U; ++k) {
             if (i + k >= n) break;
             float u, w2;
-            if (covers(m[k], t[k], u, w2) && tr != 0.0f) shade(u, w2, t[k].w, col[k], true);
+            if (covers_(m[k], t[k], u, w2) && pix.tr != 0.0f) shade(u, w2, t[k].w, col[k], true);
           }
         }
       } else {
@@ -857,12 +925,12 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
             for (int q = 0; q < 4; ++q) {
               const uint32_t j = (js >> (8 * q)) & 255u;
               float u, w2;
-              const uint32_t in = covers(s_ent[buf][0][j], s_ent[buf][1][j], u, w2) && k + q < cnt ? 1u : 0u;
+              const uint32_t in = covers_(s_ent[buf][0][j], s_ent[buf][1][j], u, w2) && k + q < cnt ? 1u : 0u;
               if (k < 32) lo |= in << (k + q);
               else hi |= in << (k + q - 32);
             }
           }
-          if (tr == 0.0f) lo = hi = 0u;
+          if (pix.tr == 0.0f) lo = hi = 0u;
           while (lo | hi) {
             uint32_t k;
             if (lo) { k = (uint32_t)__builtin_ctz(lo); lo &= lo - 1u; }
@@ -870,13 +938,13 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
             const uint32_t j = s_list[w][c + k];
             const float4 t = s_ent[buf][1][j];
             float u, w2;
-            (void)covers(s_ent[buf][0][j], t, u, w2);
+            (void)covers_(s_ent[buf][0][j], t, u, w2);
             shade(u, w2, t.w, s_ent[buf][2][j], true);
-            if (tr == 0.0f) lo = hi = 0u;
+            if (pix.tr == 0.0f) lo = hi = 0u;
           }
         }
       }
-      alive = __ballot(inside && tr != 0.0f);
+      alive = __ballot(inside && pix.tr != 0.0f);
       wave_done = alive == 0ull;
     }
     if (more) {
@@ -889,16 +957,7 @@ __global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_ras
     if (__syncthreads_and(wave_done)) break;
   }
   if (a.tile_work && v == 0 && tid == 0) a.tile_work[tile] = make_uint2(range.y - range.x, consumed);
-  if (inside) {
-    uint32_t r, g, b, al;
-    if (!f32) {
-      r = (uint32_t)c0; g = (uint32_t)c1; b = (uint32_t)c2; al = (uint32_t)tr;
-    } else {
-      r = (uint32_t)q8(c0); g = (uint32_t)q8(c1); b = (uint32_t)q8(c2); al = (uint32_t)q8(tr);
-    }
-    a.pixels[v][(size_t)py * W + px] = a.bgra ? (b | (g << 8) | (r << 16) | (al << 24))
-                                            : (r | (g << 8) | (b << 16) | (al << 24));
-  }
+  if (inside) a.pixels[v][(size_t)py * W + px] = pix.packed(a.bgra != 0);
 }
 
 void launch_bin(const BinLaunch& a, hipStream_t s) {
