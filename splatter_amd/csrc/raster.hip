@@ -643,7 +643,10 @@ void launch_tile_order(const OrderLaunch& a, hipStream_t s) {
 // (ballot + mbcnt) and blends those; a wave whose 64 pixels are all opaque
 // stops blending, and the block stops when all four have.
 template <bool UNALIGNED, bool F32, int MODE, int B>
-__global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? 6 : 5) : 1) void k_raster(RasterLaunch a) {
+// (mode 3 occupancy: 128-entry batches at seven waves per SIMD, 72 VGPRs,
+// one to three registers spilled outside the loops; the f32 blend needs
+// more and keeps six; 256-entry batches fit six at 84)
+__global__ __launch_bounds__(256, MODE == 3 ? (B == 128 ? (F32 ? 6 : 7) : 5) : 1) void k_raster(RasterLaunch a) {
   static_assert(B == 128 || B == 256, "entries staged per batch");
   constexpr int U = B / 64;  // list slots per step of the entry-major walks
   constexpr bool WALK = MODE == 1, SUBS = MODE == 2, LANES = MODE == 3;
