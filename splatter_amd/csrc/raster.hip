@@ -336,13 +336,19 @@ template <int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
   constexpr int TILE = THREADS * ITEMS;
   constexpr int WAVES = THREADS / kWave;
-  __shared__ uint32_t s_off[TILE + 1];
-  __shared__ uint32_t s_org[TILE];
-  __shared__ uint32_t s_w[TILE];
-  __shared__ uint32_t s_rec[TILE];
+  static_assert(ITEMS == 4, "a thread's items are one 16-byte LDS row");
+  // Per entry of the tile: first output slot, first tile (x | y << 16),
+  // width in tiles, record.  Thread t's four entries are one 16-byte row of
+  // each array, written with one store (consecutive threads, consecutive
+  // banks; four separate words a thread would put 8 lanes on each bank).
+  __shared__ __align__(16) uint32_t s_off[TILE + 4];
+  __shared__ __align__(16) uint32_t s_org[TILE];
+  __shared__ __align__(16) uint32_t s_w[TILE];
+  __shared__ __align__(16) uint32_t s_rec[TILE];
   // the entry that owns each output slot of a tile with at most OWN pairs
   constexpr uint32_t OWN = 4096, PER = OWN / THREADS;
-  __shared__ uint16_t s_own[OWN];
+  static_assert(PER == 16, "a thread's owner slots are two 16-byte LDS rows");
+  __shared__ __align__(16) uint16_t s_own[OWN];
   __shared__ uint32_t s_hist[kMaxPairPasses * 256];
   __shared__ uint32_t s_tmp[WAVES];
   __shared__ uint32_t s_tile;
@@ -389,33 +395,38 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
       if (r >= n || rec[k] >= n) rec[k] = 0u;  // never read past the boxes
       box[k] = boxes[rec[k]];
     }
+    uint32_t org[ITEMS], wid[ITEMS], off[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-      const uint32_t slot = tid * ITEMS + k;
-      const uint32_t r = t * TILE + slot;
+      const uint32_t r = t * TILE + tid * ITEMS + k;
       cnt[k] = 0;
+      org[k] = 0;
+      wid[k] = 0;
       if (r < n) {
-        const uint32_t c = rec[k];
         const uint2 q = box[k];
         const uint32_t lo = q.x, hi = q.y;
         const uint32_t x0 = (lo & 0xFFFFu) / kTileSize, y0 = (lo >> 16) / kTileSize;
         const uint32_t x1 = (hi & 0xFFFFu) / kTileSize, y1 = (hi >> 16) / kTileSize;
         if (lo != 0xFFFFFFFFu && x1 >= x0 && y1 >= y0) {
           cnt[k] = (x1 - x0 + 1) * (y1 - y0 + 1);
-          s_org[slot] = x0 | (y0 << 16);
-          s_w[slot] = x1 - x0 + 1;
+          org[k] = x0 | (y0 << 16);
+          wid[k] = x1 - x0 + 1;
         }
-        s_rec[slot] = c;
       }
       sum += cnt[k];
     }
+    // (an entry without pairs owns no slot, so its row is never read)
+    reinterpret_cast<uint4*>(s_org)[tid] = make_uint4(org[0], org[1], org[2], org[3]);
+    reinterpret_cast<uint4*>(s_w)[tid] = make_uint4(wid[0], wid[1], wid[2], wid[3]);
+    reinterpret_cast<uint4*>(s_rec)[tid] = make_uint4(rec[0], rec[1], rec[2], rec[3]);
     uint32_t total;
     uint32_t excl = block_exclusive_scan<THREADS>(sum, s_tmp, total);
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-      s_off[tid * ITEMS + k] = excl;
+      off[k] = excl;
       excl += cnt[k];
     }
+    reinterpret_cast<uint4*>(s_off)[tid] = make_uint4(off[0], off[1], off[2], off[3]);
     if (tid < kWave) {
       const uint64_t prefix = wave_lookback64(status, t, (uint64_t)total);
       if (tid == 0) {
@@ -435,26 +446,24 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
     // forward, so the expansion reads a slot's owner instead of
     // binary-searching the offsets (ten dependent LDS reads).  A tile with
     // more than OWN pairs searches.
+    // (Slots past the total hold stale owners: the scan carries them only
+    // forward, into slots that no pair reads.)
     const bool table = total <= OWN;
     if (table) {
       const uint32_t lane = tid & (kWave - 1), wv = tid / kWave;
-      for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t q = tid * PER + k;
-        if (q < total) s_own[q] = 0;
-      }
+      uint4* const own4 = reinterpret_cast<uint4*>(s_own);  // eight slots a row
+      for (uint32_t q = tid; q < (total + 7u) / 8u; q += THREADS) own4[q] = make_uint4(0u, 0u, 0u, 0u);
       __syncthreads();
 #pragma unroll
-      for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t j = tid * ITEMS + k;
-        const uint32_t st = s_off[j];
-        if (s_off[j + 1] > st) s_own[st] = (uint16_t)j;
-      }
+      for (int k = 0; k < ITEMS; ++k)
+        if (cnt[k] > 0u) s_own[off[k]] = (uint16_t)(tid * ITEMS + k);
       __syncthreads();
+      // this thread's sixteen slots, two rows; slot 2i + h is half h of word i
+      const uint4 ra = own4[2 * tid], rb = own4[2 * tid + 1];
+      uint32_t wd[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
       uint32_t run = 0;
-      for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t q = tid * PER + k;
-        if (q < total) run = max(run, (uint32_t)s_own[q]);
-      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) run = max(run, max(wd[i] & 0xFFFFu, wd[i] >> 16));
       uint32_t incl = run;
 #pragma unroll
       for (int o = 1; o < kWave; o <<= 1) {
@@ -466,13 +475,14 @@ __global__ __launch_bounds__(THREADS) void k_bin(BinLaunch a) {
       uint32_t carry = __shfl_up(incl, 1, kWave);
       if (lane == 0) carry = 0;
       for (uint32_t i = 0; i < wv; ++i) carry = max(carry, s_tmp[i]);
-      for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t q = tid * PER + k;
-        if (q < total) {
-          carry = max(carry, (uint32_t)s_own[q]);
-          s_own[q] = (uint16_t)carry;
-        }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const uint32_t lo = max(carry, wd[i] & 0xFFFFu);
+        carry = max(lo, wd[i] >> 16);
+        wd[i] = lo | (carry << 16);
       }
+      own4[2 * tid] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+      own4[2 * tid + 1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
       __syncthreads();
     }
     for (uint32_t s = tid; s < total; s += THREADS) {
