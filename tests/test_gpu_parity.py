@@ -165,6 +165,25 @@ def test_wide_frame_uses_pixel_boxes():
     check_case(case, conics=False)
 
 
+@pytest.mark.parametrize("scale,search", [(1.0, False), (12.0, True)])
+def test_bin_owner_table_and_search(scale, search):
+    """The binning expands a block's 1024 entries (in depth order) through an
+    owner table when they emit at most 4096 pairs, and by binary search over
+    their offsets when they emit more.  More pairs than 4096 per block of
+    the frame put at least one block above 4096; fewer than four per splat
+    put at least one at or below it.  Either way the frame equals the
+    oracle's."""
+    say(f"splat scale {scale}")
+    case = Case(Workload("gaussian", 60_000, seed=21), config(splat_scale=scale), 640, 480,
+                sphere_pose(21, radius=4))
+    check_case(case, conics=False)
+    pairs, visible = case.renderer.pair_count(), case.renderer.visible_count()
+    say(f"  pairs {pairs}, visible {visible}, {pairs / max(visible, 1):.2f} a splat")
+    assert visible > 4096
+    blocks = -(-visible // 1024)
+    assert (pairs > 4096 * blocks) if search else (pairs < 4 * visible)
+
+
 def test_edge_cases_empty_culled_tiny_ragged():
     # empty scene
     case = Case(Workload("gaussian", 0, seed=0), config(max_splat_count=1), 64, 48, sphere_pose(0))
